@@ -1,0 +1,102 @@
+"""ctypes binding of libtswrtm_hip.so (include/tswrtm.h).
+
+The library is the only compute path: if it is missing, or no gfx950 device
+is present, calls raise instead of falling back to anything else.  This plays
+the role of the module-level ``ctypes.CDLL`` load in tsw_rtm/wave_solver.py:14-71.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libtswrtm_hip.so")
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+F = ctypes.c_float
+D = ctypes.c_double
+LL = ctypes.c_longlong
+
+# name -> (restype, argtypes); mirrors include/tswrtm.h
+SIGNATURES = {
+    "acoustic_2d_step": (None, [P] * 5 + [I, I, F, F, F, I]),
+    "elastic_2d_step": (None, [P] * 10 + [I, I, F, F, F, I]),
+    "correlate_imaging_condition": (None, [P, P, P, I, I, F]),
+    "build_cerjan_sponge": (None, [P, I, I, I, F, I]),
+    "tsw_last_error": (ctypes.c_char_p, []),
+    "tsw_version": (I, []),
+    "tsw_init": (I, [I]),
+    "tsw_debug_fail_next": (None, [I]),
+    "tsw_grid_create": (I, [I, I, D, D, D, I, D, I, ctypes.POINTER(P)]),
+    "tsw_grid_destroy": (I, [P]),
+    "tsw_grid_sponge": (I, [P, P]),
+    "tsw_grid_set_sponge": (I, [P, P]),
+    "tsw_forward_acoustic": (I, [P, P, P, I, I, I, P, I, I, I, P, P]),
+    "tsw_backward_correlate": (I, [P, P, P, I, P, I, I, P, LL, P, I]),
+    "tsw_migrate_shot": (I, [P, P, P, P, I, I, I, P, I, P, I, I, I, P, P]),
+    "tsw_migrate_shots": (I, [P, P, P, I, P, P, P, P, P, P, P, I, I, I, I, P, P, P]),
+    "tsw_laplacian": (I, [P, P, I, I, D, D]),
+}
+
+
+class TswError(RuntimeError):
+    """A coarse entry point returned a negative status."""
+
+    def __init__(self, status, message):
+        super().__init__(f"tsw status {status}: {message}")
+        self.status = status
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """The loaded engine library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise FileNotFoundError(
+                        f"HIP engine not built: {LIB_PATH} (run `python -m das_amd.build`)")
+                L = ctypes.CDLL(LIB_PATH)
+                for name, (res, args) in SIGNATURES.items():
+                    fn = getattr(L, name)
+                    fn.restype = res
+                    fn.argtypes = args
+                _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().tsw_last_error().decode()
+
+
+def check(status):
+    """Raise the Python exception matching a negative engine status."""
+    if status == 0:
+        return
+    msg = last_error()
+    if status == -1:
+        raise ValueError(msg)
+    if status == -3:
+        raise MemoryError(msg)
+    raise TswError(status, msg)
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def f32(a):
+    """C-contiguous float32 view or copy (the caller keeps it alive)."""
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)

@@ -1,0 +1,67 @@
+// Host-side engine objects shared by the C-ABI translation units.
+#pragma once
+#include <mutex>
+#include <vector>
+
+#include "acoustic.h"
+
+namespace tsw {
+
+// Owning device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // (Re)allocates when the current allocation is smaller than `nbytes`.
+  int ensure(size_t nbytes);
+  float* f() const { return static_cast<float*>(p); }
+  int* i() const { return static_cast<int*>(p); }
+};
+
+// Process-wide HIP state: the device and stream the engine runs on.
+struct Ctx {
+  int device = -1;
+  hipStream_t stream = nullptr;
+};
+int ctx_get(Ctx** out);  // lazily binds to the current device and creates a stream
+
+// Receivers on one row, as a CSR map column -> receiver indices (in the
+// caller's order, so duplicate columns accumulate in that order).
+struct RecvMap {
+  std::vector<int> off, idx;
+  int build(const int* rix, int nrec, int nx);
+};
+
+// One propagation grid (tsw_rtm/wave_solver.py:89-117): constants, the
+// sponge, three rotating wavefield buffers and per-model coefficient arrays.
+struct Grid {
+  int nx = 0, nz = 0, n_sponge = 0, fs = 1;
+  double dx = 0, dz = 0, dt = 0, alpha = 0;
+  float dxf = 0, dzf = 0, dtf = 0, rdx2 = 0, rdz2 = 0, dt2 = 0, inj = 0;
+  DevBuf sponge, table, buf[3], coef[2], vel, roff, ridx, src, data, rec, snaps, illum, image,
+      isnap;
+  long long cells() const { return (long long)nx * nz; }
+  AcuStep base() const;  // geometry + constants, no buffers bound
+};
+
+int migrate_shot_dev(Grid* g, const float* d_src, int nt_src, int six, int siz,
+                     const float* d_data, int nt, int nrec, int riz, int k, float* d_image,
+                     float* d_illum, hipStream_t st);
+int postprocess_dev(Grid* g, float* stacked, float* illum, int mute, float* norm_out,
+                    float* lap_out, unsigned* maxbits, hipStream_t st);
+std::recursive_mutex& api_mutex();
+
+int grid_init(Grid* g, int nx, int nz, double dx, double dz, double dt, int n_sponge,
+              double alpha, int fs);
+// Host table expf(-alpha*d*d), d = 0..n, as wave2d.c:258-262 evaluates it.
+void sponge_table(std::vector<float>& t, int n, float alpha);
+
+}  // namespace tsw

@@ -1,0 +1,171 @@
+"""Seeded synthetic workloads for the RTM path (host-side input generation).
+
+Models and data follow the documented benchmark configurations: a two-layer
+1500/2500 m/s model cut by a 60-degree dipping fault with 50 m throw and a
+smoothed +-3% perturbation, a migration model smoothed over 100 m, a Ricker
+source and a surface receiver line.  Gaussian smoothing uses
+scipy.ndimage.gaussian_filter with sigma in metres / dx; randomness uses
+numpy.random.default_rng(seed).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+from scipy import ndimage, signal
+
+from .wave_solver import ricker_wavelet
+
+
+def faulted_two_layer(nz, nx, dx, seed=0, x_offset_m=0.0, v_top=1500.0, v_bot=2500.0,
+                      depth_m=500.0, fault_x_m=800.0, dip_deg=60.0, throw_m=50.0,
+                      perturb=0.03, corr_m=25.0):
+    """Two layers cut by a dipping normal fault (hanging wall to the right of
+    the fault trace), times a smoothed multiplicative perturbation."""
+    z = (np.arange(nz) * dx)[:, None]
+    x = (np.arange(nx) * dx + x_offset_m)[None, :]
+    # fault plane: x_f(z) = fault_x + z / tan(dip)
+    xf = fault_x_m + z / np.tan(np.deg2rad(dip_deg))
+    interface = np.where(x > xf, depth_m, depth_m - throw_m)
+    v = np.where(z >= interface, v_bot, v_top).astype(np.float64)
+    rng = np.random.default_rng(seed)
+    noise = ndimage.gaussian_filter(rng.standard_normal((nz, nx)), corr_m / dx)
+    noise /= max(noise.std(), 1e-12)
+    v *= 1.0 + np.clip(0.01 * noise, -perturb, perturb)
+    return v.astype(np.float32)
+
+
+def smooth(v, sigma_m, dx):
+    return ndimage.gaussian_filter(np.asarray(v, np.float64), sigma_m / dx).astype(np.float32)
+
+
+def bandpass(traces, dt, f_lo=0.8, f_hi=4.5, order=4):
+    """Zero-phase Butterworth band-pass along time (axis 0)."""
+    sos = signal.butter(order, [f_lo, f_hi], btype="band", fs=1.0 / dt, output="sos")
+    return signal.sosfiltfilt(sos, np.asarray(traces, np.float64), axis=0).astype(np.float32)
+
+
+@dataclass
+class Shot:
+    """One virtual shot on its own (possibly cropped) grid."""
+    nx: int
+    nz: int
+    nt: int
+    vel_fwd: np.ndarray
+    vel_bwd: np.ndarray
+    src: np.ndarray
+    six: int
+    siz: int
+    data: np.ndarray            # (nt, nrec)
+    rix: np.ndarray
+    riz: int
+    x0: int = 0                 # window offset in the global image (columns)
+
+    @property
+    def cells(self):
+        return self.nx * self.nz
+
+
+@dataclass
+class Config0:
+    """Plain acoustic RTM, one virtual shot: 200 x 400 at 5 m, dt 0.5 ms, nt 4000."""
+    nz: int = 200
+    nx: int = 400
+    dx: float = 5.0
+    dt: float = 5e-4
+    nt: int = 4000
+    n_sponge: int = 30
+    k: int = 2
+    f0: float = 2.5
+    seed: int = 0
+    true_vel: np.ndarray = field(default=None, repr=False)
+    mig_vel: np.ndarray = field(default=None, repr=False)
+    src: np.ndarray = field(default=None, repr=False)
+    rix: np.ndarray = field(default=None, repr=False)
+    six: int = 200
+    siz: int = 2
+    riz: int = 2
+
+    def __post_init__(self):
+        self.true_vel = faulted_two_layer(self.nz, self.nx, self.dx, self.seed)
+        self.mig_vel = smooth(self.true_vel, 100.0, self.dx)
+        self.src, _ = ricker_wavelet(self.f0, self.dt, self.nt * self.dt)
+        self.rix = np.arange(200, dtype=np.int32) * 2
+
+
+def make_data_config0(cfg: Config0, solver):
+    """Receiver data: forward-modelled in the true model, band-passed 0.8-4.5 Hz."""
+    rec, _ = solver.forward_propagate_acoustic(cfg.true_vel, cfg.src, cfg.six, cfg.siz, cfg.rix,
+                                               cfg.riz)
+    return bandpass(rec, cfg.dt)
+
+
+def correlogram_gather(rng, nt, dt, offsets_m, v_app=1800.0, f0=2.5, f_lo=0.8, f_hi=4.5):
+    """Whitened, 1-bit-normalised synthetic cross-correlograms (nt, nrec):
+    band-limited noise correlated against a Ricker arrival delayed by
+    offset / v_app, then sign() and spectral whitening inside the band."""
+    nrec = len(offsets_m)
+    t = np.arange(nt) * dt
+    nf = 1 << int(np.ceil(np.log2(2 * nt)))
+    freqs = np.fft.rfftfreq(nf, dt)
+    band = (freqs >= f_lo) & (freqs <= f_hi)
+    noise = rng.standard_normal((nf, nrec))
+    N = np.fft.rfft(noise, axis=0)
+    N[~band] = 0.0
+    arg = (np.pi * f0 * (t[:, None] - 1.0 / f0 - np.abs(offsets_m)[None, :] / v_app)) ** 2
+    arrival = (1.0 - 2.0 * arg) * np.exp(-arg)
+    A = np.fft.rfft(arrival, n=nf, axis=0)
+    xc = np.fft.irfft(A * np.conj(N) * N / nf, n=nf, axis=0)[:nt]
+    xc = np.sign(xc)
+    S = np.fft.rfft(xc, n=nf, axis=0)
+    S = np.where(band[:, None], S / (np.abs(S) + 1e-3 * np.abs(S).max()), 0.0)
+    g = np.fft.irfft(S, n=nf, axis=0)[:nt]
+    g /= max(np.abs(g).max(), 1e-30)
+    return (1e-3 * g).astype(np.float32)
+
+
+@dataclass
+class Config2:
+    """Batched acoustic RTM: 64 virtual shots on cropped windows nz = 300,
+    nx ~ U{600..1600}, nt ~ U{3000..6000}, dx 5 m, dt 0.5 ms, k 2."""
+    n_shots: int = 64
+    nz: int = 300
+    nx_range: tuple = (600, 1600)
+    nt_range: tuple = (3000, 6000)
+    dx: float = 5.0
+    dt: float = 5e-4
+    k: int = 2
+    n_sponge: int = 30
+    f0: float = 2.5
+    seed: int = 2
+    global_nx: int = 0
+    shots: List[Shot] = field(default_factory=list, repr=False)
+
+    def build(self, with_data=True, scale=1.0):
+        """Creates the shots.  ``scale`` < 1 shrinks nt (for quick tests)."""
+        rng = np.random.default_rng(self.seed)
+        nxs = rng.integers(self.nx_range[0], self.nx_range[1] + 1, self.n_shots)
+        nts = rng.integers(self.nt_range[0], self.nt_range[1] + 1, self.n_shots)
+        nts = np.maximum((nts * scale).astype(int), 2)
+        self.global_nx = int(nxs.max()) + 2000
+        gvel = faulted_two_layer(self.nz, self.global_nx, self.dx, self.seed,
+                                 fault_x_m=0.5 * self.global_nx * self.dx)
+        gmig = smooth(gvel, 100.0, self.dx)
+        self.shots = []
+        for i in range(self.n_shots):
+            nx, nt = int(nxs[i]), int(nts[i])
+            x0 = int(rng.integers(0, self.global_nx - nx + 1))
+            six = int(rng.integers(nx // 4, 3 * nx // 4))
+            rix = (np.arange(nx // 2) * 2).astype(np.int32)
+            src, _ = ricker_wavelet(self.f0, self.dt, nt * self.dt)
+            data = None
+            if with_data:
+                data = correlogram_gather(rng, nt, self.dt, (rix - six) * self.dx, f0=self.f0)
+            vw = np.ascontiguousarray(gmig[:, x0:x0 + nx])
+            self.shots.append(Shot(nx=nx, nz=self.nz, nt=nt, vel_fwd=vw, vel_bwd=vw, src=src,
+                                   six=six, siz=2, data=data, rix=rix, riz=2, x0=x0))
+        return self
+
+    def cell_updates(self):
+        return sum(2 * s.nt * s.cells for s in self.shots)

@@ -1,0 +1,117 @@
+/*
+ * tswrtm.h - C ABI of the MI355X (gfx950) reverse-time-migration engine.
+ *
+ * Drop-in for the native boundary of tsw_rtm (ll23002/DAS).  Loaded by
+ * ctypes from das_amd/_native.py, the way tsw_rtm/wave_solver.py:14-71 loads
+ * libwave2d.so.  Plain C types only; arrays are row-major (nz, nx) float32,
+ * flat index iz*nx + ix, exactly as in the reference.
+ *
+ * Two groups of entry points:
+ *   1. The four reference kernels with identical signatures and host-pointer,
+ *      synchronous semantics (tsw_rtm/c_engine/wave2d.c).  Each copies its
+ *      arrays to HBM, runs the HIP kernel and copies the result back.  They
+ *      return void like the reference; tsw_last_error() is non-empty after a
+ *      failed call.
+ *   2. Coarse entry points that keep the time loop, snapshots and images in
+ *      HBM.  They return 0 on success and a negative status on failure
+ *      (-1 bad argument, -2 HIP error, -3 out of device memory / budget,
+ *      -4 wrong state, -5 injected failure); tsw_last_error() describes it.
+ *      Host pointers are borrowed for the duration of the call.
+ */
+#ifndef TSWRTM_H
+#define TSWRTM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSW_ABI_VERSION 1
+
+typedef struct tsw_grid_s* tsw_grid_t;
+
+/* ------------------------------------------------------------------ 1. */
+
+/* Replaces acoustic_2d_step, tsw_rtm/c_engine/wave2d.c:19-31. */
+void acoustic_2d_step(const float* p_curr, const float* p_prev, float* p_next, const float* vel,
+                      const float* sponge, int nx, int nz, float dx, float dz, float dt,
+                      int free_surface_top);
+
+/* Replaces elastic_2d_step, tsw_rtm/c_engine/wave2d.c:95-112.  The row above
+ * the grid that the reference reads at iz = 1 with a free surface
+ * (wave2d.c:135, :170; out of bounds there) reads as 0 here. */
+void elastic_2d_step(float* vx, float* vz, float* sxx, float* szz, float* sxz, const float* lambda,
+                     const float* mu, const float* buo_x, const float* buo_z, const float* sponge,
+                     int nx, int nz, float dx, float dz, float dt, int free_surface_top);
+
+/* Replaces correlate_imaging_condition, tsw_rtm/c_engine/wave2d.c:218-225. */
+void correlate_imaging_condition(const float* src_wavefield, const float* rec_wavefield,
+                                 float* image, int nx, int nz, float dt);
+
+/* Replaces build_cerjan_sponge, tsw_rtm/c_engine/wave2d.c:237-244. */
+void build_cerjan_sponge(float* sponge, int nx, int nz, int n_sponge, float alpha,
+                         int free_surface_top);
+
+/* ------------------------------------------------------------------ 2. */
+
+/* Last error message of the calling thread ("" if none). No reference
+ * counterpart: the reference has no error channel (wave2d.c returns void). */
+const char* tsw_last_error(void);
+/* ABI version (TSW_ABI_VERSION). */
+int tsw_version(void);
+/* Binds the engine to HIP device `device` (-1: keep the current one). */
+int tsw_init(int device);
+/* Test hook: the next `count` coarse calls fail with -5 before doing work. */
+void tsw_debug_fail_next(int count);
+
+/* WaveSolver2D.__init__ (tsw_rtm/wave_solver.py:89-117): grid constants and
+ * the Cerjan sponge, built on the device. */
+int tsw_grid_create(int nx, int nz, double dx, double dz, double dt, int n_sponge,
+                    double sponge_alpha, int free_surface_top, tsw_grid_t* out);
+int tsw_grid_destroy(tsw_grid_t grid);
+/* Copies the grid's sponge (nz*nx floats) to host `out` (WaveSolver2D.sponge). */
+int tsw_grid_sponge(tsw_grid_t grid, float* out);
+/* Replaces the grid's sponge by host `sponge` (nz*nx floats).  The reference
+ * reads WaveSolver2D.sponge on every step (wave_solver.py:167), so the Python
+ * layer re-uploads it when a caller has modified that array. */
+int tsw_grid_set_sponge(tsw_grid_t grid, const float* sponge);
+
+/* WaveSolver2D.forward_propagate_acoustic (tsw_rtm/wave_solver.py:128-186).
+ * rec_out: (nt, nrec); snaps_out: NULL or ((nt-1)/k + 1, nz, nx). */
+int tsw_forward_acoustic(tsw_grid_t grid, const float* vel, const float* src, int nt, int six,
+                         int siz, const int* rix, int nrec, int riz, int k, float* rec_out,
+                         float* snaps_out);
+
+/* WaveSolver2D.backward_propagate_and_correlate (tsw_rtm/wave_solver.py:188-242).
+ * data: (nt, nrec); snaps: (nsnap, nz, nx); image_inout accumulated in place. */
+int tsw_backward_correlate(tsw_grid_t grid, const float* vel, const float* data, int nt,
+                           const int* rix, int nrec, int riz, const float* snaps, long long nsnap,
+                           float* image_inout, int k);
+
+/* TSWRTMEngine.migrate_shot (tsw_rtm/rtm_engine.py:49-104): forward in
+ * vel_fwd with snapshots kept in HBM, illumination, back-propagation in
+ * vel_bwd.  image_out and illum_out: (nz, nx). */
+int tsw_migrate_shot(tsw_grid_t grid, const float* vel_fwd, const float* vel_bwd,
+                     const float* src, int nt_src, int six, int siz, const float* data, int nt,
+                     const int* rix, int nrec, int riz, int k, float* image_out,
+                     float* illum_out);
+
+/* TSWRTMEngine.migrate_multiple_shots (tsw_rtm/rtm_engine.py:106-171):
+ * every shot migrated as tsw_migrate_shot, images and illuminations stacked
+ * in FP32 in shot order, optional sponge mute, illumination normalisation and
+ * Laplacian post-filter.  srcs[i]: nt_srcs[i] samples; datas[i]: (nts[i], nrec).
+ * Outputs (nz, nx): raw (stacked) image, Laplacian image, normalised image. */
+int tsw_migrate_shots(tsw_grid_t grid, const float* vel_fwd, const float* vel_bwd, int nshot,
+                      const float* const* srcs, const int* nt_srcs, const int* sixs,
+                      const int* sizs, const float* const* datas, const int* nts, const int* rix,
+                      int nrec, int riz, int k, int mute, float* raw_out, float* lap_out,
+                      float* norm_out);
+
+/* apply_laplacian_filter (tsw_rtm/rtm_engine.py:27-38): 5-point Laplacian with
+ * symmetric boundary, bit-identical to scipy.signal.convolve2d(.., 'same',
+ * 'symm') with the reference's float32 kernel. */
+int tsw_laplacian(const float* image, float* out, int nx, int nz, double dx, double dz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TSWRTM_H */

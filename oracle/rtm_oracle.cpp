@@ -398,7 +398,14 @@ void ora_laplacian_f32(const float* img, float* out, int nx, int nz, double dx, 
       const float lf = img[(size_t)iz * nx + cl(ix - 1, nx)];
       const float rt = img[(size_t)iz * nx + cl(ix + 1, nx)];
       const float c = img[(size_t)iz * nx + ix];
-      out[(size_t)iz * nx + ix] = wz * (up + dn) + wx * (lf + rt) + wc * c;
+      // FP32 sum in scipy.signal.convolve2d's window order (bottom, right,
+      // centre, left, top), checked bit-exact against scipy in the tests
+      float acc = dn * wz;
+      acc = acc + rt * wx;
+      acc = acc + c * wc;
+      acc = acc + lf * wx;
+      acc = acc + up * wz;
+      out[(size_t)iz * nx + ix] = acc;
     }
 }
 
