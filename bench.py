@@ -1,0 +1,159 @@
+#!/usr/bin/env python3
+"""Benchmark: batched acoustic RTM of 64 virtual shots of uneven size on one MI355X.
+
+Workload (one "step" = migrating the whole batch once): 64 virtual shots on
+aperture-cropped windows nz = 300, nx ~ U{600..1600} at dx = 5 m, nt ~
+U{3000..6000} at dt = 0.5 ms, imaging every k = 2 steps, per-shot images
+stacked in FP32.  Inputs are seeded synthetic data uploaded to HBM before the
+timed region.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Gcell-updates/s + migrated shots/s (batched/served vs plain GPU), "
+          "p50/p99 latency")
+BYTES_PER_CELL_UPDATE = 26.0  # acoustic RTM at k=2: 20 B step + 24 B per imaging step / (2k)
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, budget_cell_updates):
+    """The reference engine (wave2d.c compiled from source, oracle/_ref) driven
+    by the oracle's C time loop, on the first shots of the same workload."""
+    import numpy as np
+
+    from oracle import oracle as ora
+
+    kind = "reference" if ora.ref_lib() is not None else "port"
+    done, t, used = 0, 0.0, []
+    for i, s in enumerate(cfg.shots):
+        g = ora.Grid(s.nx, s.nz, cfg.dx, cfg.dx, cfg.dt, cfg.n_sponge)
+        t0 = time.perf_counter()
+        ora.migrate_shot(g, s.vel_fwd, s.vel_bwd, s.src, s.six, s.siz, s.data, s.rix, s.riz,
+                         cfg.k, use_reference_kernels=(kind == "reference"))
+        t += time.perf_counter() - t0
+        done += 2 * s.nt * s.nx * s.nz
+        used.append(f"shot {i} ({s.nz}x{s.nx}, nt={s.nt})")
+        log(f"cpu baseline: {used[-1]} {t:.1f}s")
+        if done >= budget_cell_updates:
+            break
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(done / t / 1e9, 4), "unit": "Gcell-updates/s", "cores": cores,
+            "kind": kind,
+            "sample": (f"{len(used)} of 64 shots: " + "; ".join(used) +
+                       f"; {done / 1e9:.2f} G cell-updates in {t:.1f} s, per-step kernels "
+                       "and snapshots in host RAM as the reference")}
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the step kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
+    p = os.path.join(ROOT, "profiles", "batch_step_config2_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots", type=int, default=64)
+    ap.add_argument("--slots", type=int, default=16)
+    ap.add_argument("--graph-steps", type=int, default=100)
+    ap.add_argument("--kernel", type=int, default=0,
+                    help="0: one launch per time step; c >= 1: temporally blocked config c")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=2.5, help="G cell-updates")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU design "
+              "(shots sharded across GPUs, one image reduce) is described in DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from das_amd import _native as N
+    from das_amd.batch import ShotBatch
+    from das_amd.workloads import Config2
+
+    N.check(N.lib().tsw_init(0))
+    t0 = time.time()
+    cfg = Config2(n_shots=args.shots).build(with_data=True)
+    log(f"workload built in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    batch = ShotBatch(cfg.shots, cfg.dx, cfg.dx, cfg.dt, n_sponge=cfg.n_sponge, k=cfg.k,
+                      slots=args.slots, graph_steps=args.graph_steps, kernel=args.kernel)
+    info = batch.info()
+    log(f"batch resident in HBM in {time.time() - t0:.1f}s: {info}")
+
+    for _ in range(args.warmup):
+        batch.run("batched")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        batch.run("batched")
+        log(f"step {i} done")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    cu = info["cell_updates"]
+    value = cu * args.steps / elapsed / 1e9
+    shots_per_s = cfg.n_shots * args.steps / elapsed
+
+    kernel_ms, launches = batch.time_kernels()
+    achieved = BYTES_PER_CELL_UPDATE * cu / (kernel_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles()
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched acoustic RTM: 64 virtual shots, nz=300, "
+                               "nx~U{600..1600}, nt~U{3000..6000}, dx=5 m, dt=0.5 ms, k=2, "
+                               "FP32 stack",
+                   "global_batch": cfg.n_shots, "cell_updates_per_step": cu,
+                   "slots": args.slots, "graph_steps": args.graph_steps,
+                   "kernel": args.kernel,
+                   "device_bytes": info["device_bytes"]},
+        "shots_per_s": round(shots_per_s, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "k_tb" if args.kernel else "k_batch_step",
+                     "launches": launches,
+                     "avg_launch_us": round(kernel_ms * 1e3 / max(launches, 1), 3),
+                     "algorithmic_bytes_per_cell_update": BYTES_PER_CELL_UPDATE},
+    }
+    if not args.no_plain:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch.run("plain")
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0
+        out["plain_gpu"] = {"value": round(cu / tp / 1e9, 3), "unit": "Gcell-updates/s",
+                            "shots_per_s": round(cfg.n_shots / tp, 3),
+                            "mode": "one shot at a time, one launch per time step"}
+        out["speedup_vs_plain"] = round(shots_per_s / (cfg.n_shots / tp), 2)
+    batch.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget * 1e9)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

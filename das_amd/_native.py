@@ -40,7 +40,27 @@ SIGNATURES = {
     "tsw_migrate_shot": (I, [P, P, P, P, I, I, I, P, I, P, I, I, I, P, P]),
     "tsw_migrate_shots": (I, [P, P, P, I, P, P, P, P, P, P, P, I, I, I, I, P, P, P]),
     "tsw_laplacian": (I, [P, P, I, I, D, D]),
+    "tsw_batch_create": (I, [I, P, D, D, D, I, D, I, I, I, I, I, ctypes.POINTER(P)]),
+    "tsw_kernel_configs": (I, []),
+    "tsw_kernel_config": (I, [I] + [ctypes.POINTER(I)] * 4),
+    "tsw_batch_run": (I, [P, I]),
+    "tsw_batch_time_kernels": (I, [P, ctypes.POINTER(D), ctypes.POINTER(I)]),
+    "tsw_batch_shot_result": (I, [P, I, P, P]),
+    "tsw_batch_stack": (I, [P, I, I, P, P]),
+    "tsw_batch_info": (I, [P, ctypes.POINTER(LL), ctypes.POINTER(LL), ctypes.POINTER(I),
+                           ctypes.POINTER(I)]),
+    "tsw_batch_destroy": (I, [P]),
 }
+
+RUN_BATCHED = 0
+RUN_PLAIN = 1
+
+
+class ShotDesc(ctypes.Structure):
+    """tsw_shot_desc of include/tswrtm.h."""
+    _fields_ = [("nx", I), ("nz", I), ("nt", I), ("vel_fwd", P), ("vel_bwd", P), ("src", P),
+                ("six", I), ("siz", I), ("data", P), ("rix", P), ("nrec", I), ("riz", I),
+                ("x0", I), ("z0", I)]
 
 
 class TswError(RuntimeError):

@@ -9,74 +9,11 @@
 
 namespace tsw {
 
-// Record / inject at cell (iz, ix) whose freshly stepped value is `v`.
-// Returns true when an injection changed `v` (the cell must be written).
-__device__ __forceinline__ bool acu_point_io(const AcuStep& s, int iz, int ix, float& v) {
-  bool touched = false;
-  if (iz == s.riz && s.roff != nullptr) {
-    const int a = s.roff[ix], b = s.roff[ix + 1];
-    if (a < b) {
-      if (s.rec_it >= 0) {
-        float* row = s.rec + (long long)s.rec_it * s.nrec;
-        for (int r = a; r < b; ++r) row[s.ridx[r]] = v;
-      }
-      if (s.rdata_it >= 0) {
-        const float* row = s.rdata + (long long)s.rdata_it * s.nrec;
-        for (int r = a; r < b; ++r) v = v + row[s.ridx[r]] * s.inj;  // receiver order
-        touched = true;
-      }
-    }
-  }
-  if (s.src_it >= 0 && iz * s.nx + ix == s.src_cell) {
-    v = v + s.src[s.src_it] * s.inj;
-    touched = true;
-  }
-  return touched;
-}
-
 __global__ __launch_bounds__(kAcuBX* kAcuBZ) void k_acoustic_step(AcuStep s) {
   const int ix = blockIdx.x * kAcuBX + threadIdx.x;
   const int iz = blockIdx.y * kAcuBZ + threadIdx.y;
   if (ix >= s.nx || iz >= s.nz) return;
-  const int nx = s.nx;
-  const int c = iz * nx + ix;
-  const float pc = s.cur[c];
-
-  // side work on the post-injection, pre-step level
-  if (s.snap != nullptr) {
-    s.snap[c] = pc;
-    s.illum[c] = s.illum[c] + pc * pc;
-  }
-  if (s.image != nullptr) s.image[c] = s.image[c] + (s.isnap[c] * pc) * s.w;
-
-  // rows 0/1 under a free surface are written by the row-2 thread of the column
-  if (s.fs && iz < 2) return;
-
-  const bool interior = iz >= 2 && iz < s.nz - 2 && ix >= 2 && ix < nx - 2;
-  float v;
-  if (interior) {
-    const float* p = s.cur;
-    float lx = TSW_C2 * (p[c - 2] + p[c + 2]);
-    lx = lx + TSW_C1 * (p[c - 1] + p[c + 1]);
-    lx = (lx + TSW_C0 * pc) * s.rdx2;
-    float lz = TSW_C2 * (p[c - 2 * nx] + p[c + 2 * nx]);
-    lz = lz + TSW_C1 * (p[c - nx] + p[c + nx]);
-    lz = (lz + TSW_C0 * pc) * s.rdz2;
-    v = (2.0f * pc - s.prv[c]) + s.coef[c] * (lx + lz);
-    if (s.sponge != nullptr) v = v * s.sponge[c];
-  } else {
-    v = s.nxt[c];  // never written by the step: keeps the buffer's old content
-  }
-  if (s.fs && iz == 2) {
-    float v0 = 0.0f;
-    acu_point_io(s, 0, ix, v0);
-    s.nxt[ix] = v0;
-    float v1 = -v;
-    acu_point_io(s, 1, ix, v1);
-    s.nxt[nx + ix] = v1;
-  }
-  const bool injected = acu_point_io(s, iz, ix, v);
-  if (interior || injected) s.nxt[c] = v;
+  acu_cell(s, ix, iz);
 }
 
 // Injection only (no step, no recording), into s.nxt.

@@ -1,0 +1,269 @@
+// Temporally blocked acoustic RTM step kernel: one launch advances T time
+// steps of every active shot.
+//
+// A workgroup owns a core tile of W x H cells and loads it with a 2T-cell
+// halo (clipped at the grid).  Thread (lx, ly) owns column X0+lx and the R
+// consecutive rows Z0+ly*R .. Z0+ly*R+R-1 of the extended tile; it keeps the
+// three most recent time levels of its cells, the model coefficient, the
+// sponge factor and the image / illumination accumulator in registers.
+// Neighbours along x, and along z across thread boundaries, are exchanged
+// through double-buffered LDS with one barrier per sub-step.  After s sub-steps
+// the cells whose distance to a cut (non-grid) tile edge is >= 2s are exact,
+// so after T sub-steps the core is exact and is written back.
+//
+// Per sub-step the kernel reproduces the reference loop exactly
+// (tsw_rtm/wave_solver.py:157-183 forward, :210-242 backward,
+// tsw_rtm/c_engine/wave2d.c:19-82): snapshot + illumination or zero-lag
+// imaging of the post-injection level, the 4th-order update with the
+// reference's FP32 association, free-surface rows (row 0 = 0, row 1 = -row 2,
+// all three rows owned by one thread), cells outside [2,nz-2) x [2,nx-2) that
+// keep the level from three buffers back, and injection of the next
+// iteration's samples.  Results are bit-identical to the per-step kernel.
+#pragma once
+#include "batch.h"
+
+namespace tsw {
+
+// One block of sub-steps of one shot in one launch.
+struct TbEntry {
+  int shot;
+  int phase;    // 0 forward (iterations it0, it0+1, ...), 1 back-propagation (it0, it0-1, ...)
+  int it0;
+  int nsub;     // sub-steps in this block (<= T)
+  int flags;    // kTbFirst: state generated (zero + iteration-0 injection), not loaded
+                // kTbLast: last block of the phase (no state write-back; illum scaled)
+  int rd;       // state set read (0/1)
+  int tiles_x;
+  int pad;
+};
+constexpr int kTbFirst = 1;
+constexpr int kTbLast = 2;
+
+// State of a shot for the blocked kernel: two sets of three levels
+// (level t-2, t-1, t) used alternately as input and output.
+struct TbShot {
+  int nx, nz, nt, k;
+  int src_cell, riz, nrec;
+  int pad;
+  float* set[2][3];
+  const float* coef_f;
+  const float* coef_b;
+  const float* src;
+  const float* data;
+  const int* roff;
+  const int* ridx;
+  float* snaps;
+  float* illum;
+  float* image;
+};
+
+__device__ __forceinline__ int xcd_tile_tb(int b, int n) {
+  const int per = n >> 3, rem = n & 7;
+  const int x = b & 7, l = b >> 3;
+  return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
+}
+
+template <int T, int R, int WX, int WZ>
+struct TbShape {
+  static constexpr int CX = 64 * WX;     // extended columns
+  static constexpr int CZ = WZ * R;      // extended rows
+  static constexpr int W = CX - 4 * T;   // core columns
+  static constexpr int H = CZ - 4 * T;   // core rows
+  static constexpr int LS = CX + 4;      // LDS row stride (2 pad columns each side)
+  static constexpr int LR = CZ + 4;      // LDS rows (2 pad rows each side)
+  static constexpr int THREADS = 64 * WX * WZ;
+  static_assert(W > 0 && H > 0, "halo larger than the tile");
+  static_assert(R >= 3, "rows 0..2 of a free surface must belong to one thread");
+};
+
+template <int T, int R, int WX, int WZ>
+__global__ __launch_bounds__(64 * WX * WZ) void k_tb(BatchConst bc, const TbShot* shots,
+                                                      const TbEntry* ents, const int* pre,
+                                                      int nent, int ntiles) {
+  using S = TbShape<T, R, WX, WZ>;
+  __shared__ float lds[2][S::LR * S::LS];
+
+  const int b = xcd_tile_tb(blockIdx.x, ntiles);
+  int lo = 0, hi = nent - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const TbEntry E = ents[lo];
+  const TbShot& P = shots[E.shot];
+  const int t = b - pre[lo];
+  const int nx = P.nx, nz = P.nz;
+  const int cx0 = (t % E.tiles_x) * S::W;
+  const int cz0 = (t / E.tiles_x) * S::H;
+  const int X0 = cx0 - 2 * T > 0 ? cx0 - 2 * T : 0;
+  const int Z0 = cz0 - 2 * T > 0 ? cz0 - 2 * T : 0;
+  const int lx = threadIdx.x, ly = threadIdx.y;
+  const int col = X0 + lx;
+  const int row0 = Z0 + ly * R;
+  const bool fwd = E.phase == 0;
+  const bool first = (E.flags & kTbFirst) != 0;
+  const bool last = (E.flags & kTbLast) != 0;
+  const long long ncell = (long long)nx * nz;
+
+  const bool col_in = col < nx;
+  const bool core_col = col_in && col >= cx0 && col < cx0 + S::W;
+  const bool col_int = col >= 2 && col < nx - 2;
+  // per-row classification bit masks
+  unsigned m_in = 0, m_int = 0, m_core = 0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int row = row0 + q;
+    if (col_in && row < nz) {
+      m_in |= 1u << q;
+      if (col_int && row >= 2 && row < nz - 2) m_int |= 1u << q;
+      if (core_col && row >= cz0 && row < cz0 + S::H) m_core |= 1u << q;
+    }
+  }
+  // rows 0/1 under a free surface are rewritten every step, never "kept"
+  const unsigned m_fs = (bc.fs && row0 == 0) ? (m_in & 3u) : 0u;
+  const unsigned m_keep = m_in & ~m_int & ~m_fs;
+
+  const float* coef = fwd ? P.coef_f : P.coef_b;
+  float a[R], bl[R], c[R], cf[R], sp[R], acc[R];
+
+  // ---- load (or generate) the three input levels
+  const int rd = E.rd;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int row = row0 + q;
+    a[q] = 0.0f;
+    bl[q] = 0.0f;
+    c[q] = 0.0f;
+    cf[q] = 0.0f;
+    sp[q] = 1.0f;
+    acc[q] = 0.0f;
+    if (m_in >> q & 1u) {
+      const int idx = row * nx + col;
+      cf[q] = coef[idx];
+      const int d = sponge_depth(col, row, nx, nz, bc.n_sponge, bc.fs);
+      if (d > 0) sp[q] = bc.table[d];
+      if (first) {
+        if (fwd) {
+          if (idx == P.src_cell) c[q] = 0.0f + P.src[0] * bc.inj;
+        } else if (row == P.riz) {
+          const int r0 = P.roff[col], r1 = P.roff[col + 1];
+          const float* drow = P.data + (long long)(P.nt - 1) * P.nrec;
+          float v = 0.0f;
+          for (int r = r0; r < r1; ++r) v = v + drow[P.ridx[r]] * bc.inj;
+          c[q] = v;
+        }
+      } else {
+        c[q] = P.set[rd][2][idx];
+        bl[q] = P.set[rd][1][idx];
+        if (m_keep >> q & 1u) a[q] = P.set[rd][0][idx];
+        if (m_core >> q & 1u) acc[q] = fwd ? P.illum[idx] : P.image[idx];
+      }
+    }
+  }
+  float* X = lds[0];
+#pragma unroll
+  for (int q = 0; q < R; ++q) X[(ly * R + q + 2) * S::LS + lx + 2] = c[q];
+  __syncthreads();
+
+  const float w = bc.w;
+  for (int j = 0; j < E.nsub; ++j) {
+    const int it = fwd ? E.it0 + j : E.it0 - j;
+    float* Y = lds[(j + 1) & 1];
+    const bool imaging = it % P.k == 0;
+    // side work on the post-injection level `c`
+    if (imaging && m_core) {
+      float* snap = P.snaps + (long long)(it / P.k) * ncell;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        if (m_core >> q & 1u) {
+          const int idx = (row0 + q) * nx + col;
+          if (fwd) {
+            snap[idx] = c[q];
+            acc[q] = acc[q] + c[q] * c[q];
+          } else {
+            acc[q] = acc[q] + (snap[idx] * c[q]) * w;
+          }
+        }
+      }
+    }
+    // leapfrog update
+    float n[R];
+    const float* xr = X + 2;  // column offset of lx = 0
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int lr = (ly * R + q + 2) * S::LS + lx;
+      float v;
+      if (m_int >> q & 1u) {
+        const float pc = c[q];
+        const float xm2 = xr[lr - 2], xm1 = xr[lr - 1], xp1 = xr[lr + 1], xp2 = xr[lr + 2];
+        const float zm2 = q >= 2 ? c[q - 2] : xr[lr - 2 * S::LS];
+        const float zm1 = q >= 1 ? c[q - 1] : xr[lr - S::LS];
+        const float zp1 = q + 1 < R ? c[q + 1] : xr[lr + S::LS];
+        const float zp2 = q + 2 < R ? c[q + 2] : xr[lr + 2 * S::LS];
+        float lx2 = TSW_C2 * (xm2 + xp2);
+        lx2 = lx2 + TSW_C1 * (xm1 + xp1);
+        lx2 = (lx2 + TSW_C0 * pc) * bc.rdx2;
+        float lz2 = TSW_C2 * (zm2 + zp2);
+        lz2 = lz2 + TSW_C1 * (zm1 + zp1);
+        lz2 = (lz2 + TSW_C0 * pc) * bc.rdz2;
+        v = (2.0f * pc - bl[q]) + cf[q] * (lx2 + lz2);
+        v = v * sp[q];
+      } else {
+        v = a[q];  // kept cells (and cells outside the grid, which stay 0)
+      }
+      n[q] = v;
+    }
+    if (m_fs) {  // this thread holds rows 0, 1 and 2 of its column
+      n[0] = 0.0f;
+      n[1] = -n[2];
+    }
+    // injection of the next iteration's samples
+    if (fwd) {
+      if (it + 1 < P.nt) {
+        const int sr = P.src_cell / nx, sc = P.src_cell - sr * nx;
+        if (sc == col && sr >= row0 && sr < row0 + R) {
+#pragma unroll
+          for (int q = 0; q < R; ++q)
+            if (row0 + q == sr) n[q] = n[q] + P.src[it + 1] * bc.inj;
+        }
+      }
+    } else if (it >= 1 && P.riz >= row0 && P.riz < row0 + R && col_in) {
+      const int r0 = P.roff[col], r1 = P.roff[col + 1];
+      if (r0 < r1) {
+        const float* drow = P.data + (long long)(it - 1) * P.nrec;
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          if (row0 + q == P.riz)
+            for (int r = r0; r < r1; ++r) n[q] = n[q] + drow[P.ridx[r]] * bc.inj;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      Y[(ly * R + q + 2) * S::LS + lx + 2] = n[q];
+      a[q] = bl[q];
+      bl[q] = c[q];
+      c[q] = n[q];
+    }
+    X = Y;
+    __syncthreads();
+  }
+
+  // ---- write back the core
+  const int wr = rd ^ 1;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    if (m_core >> q & 1u) {
+      const int idx = (row0 + q) * nx + col;
+      if (!last) {
+        P.set[wr][2][idx] = c[q];
+        P.set[wr][1][idx] = bl[q];
+        if (m_keep >> q & 1u) P.set[wr][0][idx] = a[q];
+      }
+      if (fwd) P.illum[idx] = last ? acc[q] * w : acc[q];
+      else P.image[idx] = acc[q];
+    }
+  }
+}
+
+}  // namespace tsw

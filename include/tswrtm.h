@@ -28,6 +28,24 @@ extern "C" {
 #define TSW_ABI_VERSION 1
 
 typedef struct tsw_grid_s* tsw_grid_t;
+typedef struct tsw_batch_s* tsw_batch_t;
+
+/* One virtual shot of a batch, on its own (nz, nx) grid.  The shot's grid is
+ * the window [z0, z0+nz) x [x0, x0+nx) of a larger stacking grid. */
+typedef struct tsw_shot_desc {
+  int nx, nz, nt;          /* grid and time steps (source and data both nt long) */
+  const float* vel_fwd;    /* (nz, nx) forward (sediment-flood) model */
+  const float* vel_bwd;    /* (nz, nx) back-propagation (rock-flood) model */
+  const float* src;        /* (nt) source wavelet */
+  int six, siz;            /* source cell */
+  const float* data;       /* (nt, nrec) receiver gather to back-inject */
+  const int* rix;          /* (nrec) receiver columns on row riz */
+  int nrec, riz;
+  int x0, z0;              /* window offset in the stacking grid */
+} tsw_shot_desc;
+
+#define TSW_RUN_BATCHED 0  /* all active shots in one launch per step (graphs) */
+#define TSW_RUN_PLAIN 1    /* one shot at a time, one launch per step */
 
 /* ------------------------------------------------------------------ 1. */
 
@@ -110,6 +128,37 @@ int tsw_migrate_shots(tsw_grid_t grid, const float* vel_fwd, const float* vel_bw
  * symmetric boundary, bit-identical to scipy.signal.convolve2d(.., 'same',
  * 'symm') with the reference's float32 kernel. */
 int tsw_laplacian(const float* image, float* out, int nx, int nz, double dx, double dz);
+
+/* Batched acoustic RTM (no reference counterpart; it batches what
+ * TSWRTMEngine.migrate_multiple_shots, rtm_engine.py:125-143, runs one shot at
+ * a time).  Uploads every shot's models, source and data, sizes the HBM state
+ * and captures the time loop into hipGraphs of `graph_steps` time steps
+ * (0: no graphs).  Shots are packed into `slots` concurrent lanes (<= 0: all
+ * shots at once).  kernel 0: one launch per time step over the tiles of every
+ * active shot; kernel c in [1, tsw_kernel_configs()]: temporally blocked
+ * launches of configuration c (T steps per launch).  Per-shot results equal
+ * tsw_migrate_shot's bit for bit. */
+int tsw_batch_create(int nshot, const tsw_shot_desc* shots, double dx, double dz, double dt,
+                     int n_sponge, double sponge_alpha, int free_surface_top, int k, int slots,
+                     int graph_steps, int kernel, tsw_batch_t* out);
+/* Number of temporally blocked kernel configurations, and configuration
+ * `id`'s steps per launch, core tile (W columns x H rows) and threads. */
+int tsw_kernel_configs(void);
+int tsw_kernel_config(int id, int* T, int* W, int* H, int* threads);
+/* Migrates every shot (mode TSW_RUN_BATCHED or TSW_RUN_PLAIN); synchronous. */
+int tsw_batch_run(tsw_batch_t batch, int mode);
+/* Runs the batch once with a HIP event pair around every step launch (no
+ * graphs); returns the summed kernel time and the number of launches. */
+int tsw_batch_time_kernels(tsw_batch_t batch, double* kernel_ms, int* launches);
+/* Per-shot image and illumination (nz, nx) of the last run; either may be NULL. */
+int tsw_batch_shot_result(tsw_batch_t batch, int shot, float* image_out, float* illum_out);
+/* FP32 stack, in shot order, of every shot's image and illumination into a
+ * (gnz, gnx) grid at the shots' window offsets. */
+int tsw_batch_stack(tsw_batch_t batch, int gnx, int gnz, float* image_out, float* illum_out);
+/* Cell-updates per run, device bytes held, global steps, captured graphs. */
+int tsw_batch_info(tsw_batch_t batch, long long* cell_updates, long long* device_bytes,
+                   int* total_steps, int* n_graphs);
+int tsw_batch_destroy(tsw_batch_t batch);
 
 #ifdef __cplusplus
 }
