@@ -74,7 +74,7 @@ def main():
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--graph-steps", type=int, default=100)
-    ap.add_argument("--kernel", type=int, default=0,
+    ap.add_argument("--kernel", type=int, default=7,
                     help="0: one launch per time step; c >= 1: temporally blocked config c")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
