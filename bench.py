@@ -72,10 +72,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--shots", type=int, default=64)
-    ap.add_argument("--slots", type=int, default=16)
+    ap.add_argument("--slots", type=int, default=-1, help="-1: chosen by the autotuner")
     ap.add_argument("--graph-steps", type=int, default=100)
-    ap.add_argument("--kernel", type=int, default=7,
-                    help="0: one launch per time step; c >= 1: temporally blocked config c")
+    ap.add_argument("--kernel", type=int, default=-1,
+                    help="-1: autotuned; 0: one launch per time step; c >= 1: temporally "
+                         "blocked config c")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=2.5, help="G cell-updates")
@@ -96,6 +97,17 @@ def main():
     t0 = time.time()
     cfg = Config2(n_shots=args.shots).build(with_data=True)
     log(f"workload built in {time.time() - t0:.1f}s")
+    tuning = None
+    if args.kernel < 0 or args.slots < 0:
+        from das_amd import autotune
+
+        t0 = time.time()
+        tuning = autotune.tune(cfg.shots, cfg.dx, cfg.dt, cfg.k, cfg.n_sponge, log=log)
+        log(f"autotune ({time.time() - t0:.1f}s): {tuning}")
+        if args.kernel < 0:
+            args.kernel = tuning["kernel"]
+        if args.slots < 0:
+            args.slots = tuning["slots"]
     t0 = time.time()
     batch = ShotBatch(cfg.shots, cfg.dx, cfg.dx, cfg.dt, n_sponge=cfg.n_sponge, k=cfg.k,
                       slots=args.slots, graph_steps=args.graph_steps, kernel=args.kernel)
@@ -129,6 +141,7 @@ def main():
                    "global_batch": cfg.n_shots, "cell_updates_per_step": cu,
                    "slots": args.slots, "graph_steps": args.graph_steps,
                    "kernel": args.kernel,
+                   "autotune": tuning,
                    "device_bytes": info["device_bytes"]},
         "shots_per_s": round(shots_per_s, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -31,6 +31,8 @@ SIGNATURES = {
     "tsw_version": (I, []),
     "tsw_init": (I, [I]),
     "tsw_debug_fail_next": (None, [I]),
+    "tsw_mem_info": (I, [ctypes.POINTER(LL), ctypes.POINTER(LL)]),
+    "tsw_device_name": (I, [ctypes.c_char_p, I]),
     "tsw_grid_create": (I, [I, I, D, D, D, I, D, I, ctypes.POINTER(P)]),
     "tsw_grid_destroy": (I, [P]),
     "tsw_grid_sponge": (I, [P, P]),

@@ -211,6 +211,14 @@ static const TbCfg kTbCfgs[] = {
     TB_CFG(4, 6, 2, 8),  // 6: 128 x 48, 112 x 32 core
     TB_CFG(2, 4, 2, 8),  // 7: 128 x 32, 120 x 24 core
     TB_CFG(1, 4, 2, 4),  // 8: 128 x 16, 124 x 12 core (one step per launch)
+    TB_CFG(2, 4, 1, 16),  // 9: 64 x 64, 56 x 56 core, 1024 threads
+    TB_CFG(3, 4, 1, 16),  // 10: 64 x 64, 52 x 52 core
+    TB_CFG(4, 4, 1, 16),  // 11: 64 x 64, 48 x 48 core
+    TB_CFG(3, 4, 2, 8),   // 12: 128 x 32, 116 x 20 core
+    TB_CFG(2, 6, 1, 16),  // 13: 64 x 96, 56 x 88 core
+    TB_CFG(4, 6, 1, 16),  // 14: 64 x 96, 48 x 80 core
+    TB_CFG(5, 4, 1, 16),  // 15: 64 x 64, 44 x 44 core
+    TB_CFG(6, 4, 1, 16),  // 16: 64 x 64, 40 x 40 core
 };
 constexpr int kNumTbCfgs = (int)(sizeof(kTbCfgs) / sizeof(kTbCfgs[0]));
 

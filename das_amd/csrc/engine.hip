@@ -318,6 +318,28 @@ int tsw_init(int device) {
   return ctx_get(&c);
 }
 
+int tsw_mem_info(long long* free_bytes, long long* total_bytes) {
+  std::lock_guard<std::recursive_mutex> lk(g_api_mu);
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  size_t f = 0, t = 0;
+  TSW_HIP(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = (long long)f;
+  if (total_bytes) *total_bytes = (long long)t;
+  return kOk;
+}
+
+int tsw_device_name(char* out, int len) {
+  std::lock_guard<std::recursive_mutex> lk(g_api_mu);
+  TSW_ARG(out && len > 0, "bad buffer");
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  hipDeviceProp_t prop;
+  TSW_HIP(hipGetDeviceProperties(&prop, ctx->device));
+  snprintf(out, (size_t)len, "%s:%d", prop.gcnArchName, prop.multiProcessorCount);
+  return kOk;
+}
+
 void tsw_debug_fail_next(int count) {
   std::lock_guard<std::mutex> lk(g_fail_mu);
   g_fail_pending = count;

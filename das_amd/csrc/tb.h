@@ -178,11 +178,12 @@ __global__ __launch_bounds__(64 * WX * WZ) void k_tb(BatchConst bc, const TbShot
       for (int q = 0; q < R; ++q) {
         if (m_core >> q & 1u) {
           const int idx = (row0 + q) * nx + col;
+          // snapshots stream through HBM once each way: keep them out of the caches
           if (fwd) {
-            snap[idx] = c[q];
+            __builtin_nontemporal_store(c[q], snap + idx);
             acc[q] = acc[q] + c[q] * c[q];
           } else {
-            acc[q] = acc[q] + (snap[idx] * c[q]) * w;
+            acc[q] = acc[q] + (__builtin_nontemporal_load(snap + idx) * c[q]) * w;
           }
         }
       }

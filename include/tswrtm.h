@@ -78,6 +78,10 @@ const char* tsw_last_error(void);
 int tsw_version(void);
 /* Binds the engine to HIP device `device` (-1: keep the current one). */
 int tsw_init(int device);
+/* Free and total device memory of the engine's device, in bytes. */
+int tsw_mem_info(long long* free_bytes, long long* total_bytes);
+/* "<gcnArchName>:<compute units>" of the engine's device, for tuning caches. */
+int tsw_device_name(char* out, int len);
 /* Test hook: the next `count` coarse calls fail with -5 before doing work. */
 void tsw_debug_fail_next(int count);
 
