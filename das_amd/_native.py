@@ -42,6 +42,9 @@ SIGNATURES = {
     "tsw_migrate_shot": (I, [P, P, P, P, I, I, I, P, I, P, I, I, I, P, P]),
     "tsw_migrate_shots": (I, [P, P, P, I, P, P, P, P, P, P, P, I, I, I, I, P, P, P]),
     "tsw_laplacian": (I, [P, P, I, I, D, D]),
+    "tsw_forward_elastic": (I, [P] * 6 + [I, I, I, P, I, I, I, P, P, P]),
+    "tsw_backward_correlate_elastic": (I, [P] * 6 + [I, P, I, I, P, LL, P, I]),
+    "tsw_migrate_shot_elastic": (I, [P, P, P, P, I, I, P, I, P, I, I, I, P, P]),
     "tsw_batch_create": (I, [I, P, D, D, D, I, D, I, I, I, I, I, ctypes.POINTER(P)]),
     "tsw_kernel_configs": (I, []),
     "tsw_kernel_config": (I, [I] + [ctypes.POINTER(I)] * 4),

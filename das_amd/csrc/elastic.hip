@@ -1,12 +1,14 @@
 // Elastic P-SV velocity-stress step for gfx950 (Levander staggered grid,
 // tsw_rtm/c_engine/wave2d.c:95-212).  FP32 association follows the reference;
 // built with -ffp-contract=off for bit-identical results.
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <vector>
 
 #include "elastic.h"
 #include "engine.h"
+#include "tswrtm.h"
 
 namespace tsw {
 
@@ -18,10 +20,35 @@ __device__ __forceinline__ float row_or_zero(const float* f, int i) { return i <
 __global__ __launch_bounds__(kElaBX* kElaBZ) void k_elastic_velocity(ElaStep s) {
   const int ix = blockIdx.x * kElaBX + threadIdx.x;
   const int iz = blockIdx.y * kElaBZ + threadIdx.y;
-  const int z0 = s.fs ? 1 : 2;
-  if (ix < 2 || ix >= s.nx - 2 || iz < z0 || iz >= s.nz - 2) return;
+  if (ix >= s.nx || iz >= s.nz) return;
   const int nx = s.nx;
   const int i = iz * nx + ix;
+  // driver side work on this cell's vz before its update
+  float vzc = s.vz[i];
+  bool injected = false;
+  if (s.src_it >= 0 && i == s.src_cell) {
+    vzc = vzc + (s.dt * s.bz[i]) * s.src[s.src_it];
+    injected = true;
+  }
+  if (s.rdata_it >= 0 && iz == s.riz) {
+    const int r0 = s.roff[ix], r1 = s.roff[ix + 1];
+    const float* row = s.rdata + (long long)s.rdata_it * s.nrec;
+    for (int r = r0; r < r1; ++r) {
+      vzc = vzc + (s.dt * s.bz[i]) * row[s.ridx[r]];
+      injected = true;
+    }
+  }
+  if (s.snap != nullptr) {
+    s.snap[i] = vzc;
+    s.illum[i] = s.illum[i] + vzc * vzc;
+  }
+  if (s.image != nullptr) s.image[i] = s.image[i] + (s.isnap[i] * vzc) * s.w;
+
+  const int z0 = s.fs ? 1 : 2;
+  if (ix < 2 || ix >= nx - 2 || iz < z0 || iz >= s.nz - 2) {
+    if (injected) s.vz[i] = vzc;
+    return;
+  }
   const float a = TSW_S1, b = TSW_S2;
   const float* sxx = s.sxx;
   const float* szz = s.szz;
@@ -32,7 +59,7 @@ __global__ __launch_bounds__(kElaBX* kElaBZ) void k_elastic_velocity(ElaStep s) 
   float vx = s.vx[i] + (s.dt * s.bx[i]) * (dsxx + dsxz_z);
   const float dsxz_x = (a * (sxz[i] - sxz[i - 1]) + b * (sxz[i + 1] - sxz[i - 2])) * s.rdx;
   const float dszz = (a * (szz[i + nx] - szz[i]) + b * (szz[i + 2 * nx] - szz[i - nx])) * s.rdz;
-  float vz = s.vz[i] + (s.dt * s.bz[i]) * (dsxz_x + dszz);
+  float vz = vzc + (s.dt * s.bz[i]) * (dsxz_x + dszz);
   if (s.sponge != nullptr) {
     const float g = s.sponge[i];
     vx = vx * g;
@@ -40,6 +67,14 @@ __global__ __launch_bounds__(kElaBX* kElaBZ) void k_elastic_velocity(ElaStep s) 
   }
   s.vx[i] = vx;
   s.vz[i] = vz;
+}
+
+// Vertical strain rate (4th-order staggered D-z of vz) at (iz, ix); rows
+// outside the grid read 0.
+__device__ __forceinline__ float ela_strain_rate(const ElaStep& s, int iz, int ix) {
+  const int nx = s.nx, nz = s.nz;
+  auto at = [&](int z) { return (z < 0 || z >= nz) ? 0.0f : s.vz[z * nx + ix]; };
+  return (TSW_S1 * (at(iz) - at(iz - 1)) + TSW_S2 * (at(iz + 1) - at(iz - 2))) * s.rdz;
 }
 
 __global__ __launch_bounds__(kElaBX* kElaBZ) void k_elastic_stress(ElaStep s) {
@@ -50,6 +85,14 @@ __global__ __launch_bounds__(kElaBX* kElaBZ) void k_elastic_stress(ElaStep s) {
   const float a = TSW_S1, b = TSW_S2;
   const float* vx = s.vx;
   const float* vz = s.vz;
+  if (s.rec_it >= 0 && iz == s.riz) {
+    const int r0 = s.roff[ix], r1 = s.roff[ix + 1];
+    if (r0 < r1) {
+      const float e = ela_strain_rate(s, iz, ix);
+      float* row = s.rec + (long long)s.rec_it * s.nrec;
+      for (int r = r0; r < r1; ++r) row[s.ridx[r]] = e;
+    }
+  }
   if (iz == 0) {
     if (!s.fs) return;
     // free surface (wave2d.c:198-211): assignment without dt, as the reference
@@ -129,6 +172,11 @@ int ref_elastic(float* vx, float* vz, float* sxx, float* szz, float* sxz, const 
   }
   ElaStep s;
   std::memset(&s, 0, sizeof s);
+  s.src_it = -1;
+  s.rdata_it = -1;
+  s.rec_it = -1;
+  s.src_cell = -1;
+  s.riz = -1;
   s.nx = nx;
   s.nz = nz;
   s.fs = fs ? 1 : 0;
@@ -164,3 +212,253 @@ extern "C" void elastic_2d_step(float* vx, float* vz, float* sxx, float* szz, fl
   (void)ref_elastic(vx, vz, sxx, szz, sxz, lambda, mu, buo_x, buo_z, sponge, nx, nz, dx, dz, dt,
                     free_surface_top);
 }
+
+// ------------------------------------------------------------ driver loops
+namespace tsw {
+namespace {
+
+ElaStep ela_base(Grid* g, int slot) {
+  ElaStep s;
+  std::memset(&s, 0, sizeof s);
+  s.nx = g->nx;
+  s.nz = g->nz;
+  s.fs = g->fs;
+  s.rdx = 1.0f / g->dxf;
+  s.rdz = 1.0f / g->dzf;
+  s.dt = g->dtf;
+  s.vx = g->ef[0].f();
+  s.vz = g->ef[1].f();
+  s.sxx = g->ef[2].f();
+  s.szz = g->ef[3].f();
+  s.sxz = g->ef[4].f();
+  s.lam = g->epar[slot][0].f();
+  s.mu = g->epar[slot][1].f();
+  s.bx = g->epar[slot][2].f();
+  s.bz = g->epar[slot][3].f();
+  s.sponge = g->sponge.f();
+  s.src_cell = -1;
+  s.src_it = -1;
+  s.rdata_it = -1;
+  s.rec_it = -1;
+  s.riz = -1;
+  s.roff = g->roff.i();
+  s.ridx = g->ridx.i();
+  return s;
+}
+
+int ela_upload(Grid* g, int slot, const float* lam, const float* mu, const float* bx,
+               const float* bz, hipStream_t st) {
+  const size_t nb = (size_t)g->cells() * sizeof(float);
+  const float* p[4] = {lam, mu, bx, bz};
+  for (int j = 0; j < 4; ++j) {
+    TSW_ARG(p[j], "null elastic model array");
+    TSW_TRY(g->epar[slot][j].ensure(nb));
+    TSW_HIP(hipMemcpyAsync(g->epar[slot][j].p, p[j], nb, hipMemcpyHostToDevice, st));
+  }
+  for (auto& f : g->ef) TSW_TRY(f.ensure(nb));
+  return kOk;
+}
+
+int ela_recv(Grid* g, const int* rix, int nrec, hipStream_t st) {
+  RecvMap m;
+  TSW_TRY(m.build(rix, nrec, g->nx));
+  TSW_TRY(g->roff.ensure(m.off.size() * sizeof(int)));
+  TSW_TRY(g->ridx.ensure(m.idx.size() * sizeof(int)));
+  TSW_HIP(hipMemcpyAsync(g->roff.p, m.off.data(), m.off.size() * sizeof(int),
+                         hipMemcpyHostToDevice, st));
+  TSW_HIP(hipMemcpyAsync(g->ridx.p, m.idx.data(), m.idx.size() * sizeof(int),
+                         hipMemcpyHostToDevice, st));
+  TSW_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+// Forward elastic loop: injection, side work, step, strain-rate recording.
+int ela_forward(Grid* g, int slot, const float* d_src, int nt, int six, int siz, int nrec,
+                int riz, int k, float* d_rec, float* d_snaps, float* d_illum, hipStream_t st) {
+  const long long n = g->cells();
+  for (auto& f : g->ef) launch_fill(f.f(), n, 0.0f, st);
+  ElaStep s = ela_base(g, slot);
+  s.src = d_src;
+  s.src_cell = siz * g->nx + six;
+  s.riz = riz;
+  s.nrec = nrec;
+  s.rec = d_rec;
+  for (int it = 0; it < nt; ++it) {
+    s.src_it = it;
+    s.snap = nullptr;
+    s.illum = nullptr;
+    if (d_snaps && it % k == 0) {
+      s.snap = d_snaps + (long long)(it / k) * n;
+      s.illum = d_illum;
+    }
+    s.rec_it = -1;
+    launch_elastic_velocity(s, st);
+    s.rec_it = d_rec ? it : -1;
+    launch_elastic_stress(s, st);
+  }
+  TSW_HIP(hipGetLastError());
+  return kOk;
+}
+
+int ela_backward(Grid* g, int slot, const float* d_data, int nt, int nrec, int riz,
+                 const float* d_snaps, long long nsnap, float* d_image, int k, hipStream_t st) {
+  const long long n = g->cells();
+  for (auto& f : g->ef) launch_fill(f.f(), n, 0.0f, st);
+  ElaStep s = ela_base(g, slot);
+  s.riz = riz;
+  s.nrec = nrec;
+  s.rdata = d_data;
+  s.w = (float)(g->dt * k);
+  long long cursor = (nt - 1) / k;
+  for (int it = nt - 1; it >= 0; --it) {
+    s.rdata_it = it;
+    s.image = nullptr;
+    s.isnap = nullptr;
+    if (it % k == 0 && cursor >= 0 && cursor < nsnap) {
+      s.image = d_image;
+      s.isnap = d_snaps + cursor * n;
+      --cursor;
+    }
+    launch_elastic_velocity(s, st);
+    launch_elastic_stress(s, st);
+  }
+  TSW_HIP(hipGetLastError());
+  return kOk;
+}
+
+int ela_shot_checks(Grid* g, int nt, int nrec, int riz, int k) {
+  TSW_ARG(nt >= 1, "nt must be >= 1");
+  TSW_ARG(nrec >= 0, "n_rec must be >= 0");
+  TSW_ARG(riz >= 0 && riz < g->nz, "receiver_iz %d outside [0, %d)", riz, g->nz);
+  TSW_ARG(k >= 1, "snapshot_interval must be >= 1");
+  return kOk;
+}
+
+}  // namespace
+}  // namespace tsw
+
+extern "C" {
+
+int tsw_forward_elastic(tsw_grid_t h, const float* lam, const float* mu, const float* bx,
+                        const float* bz, const float* src, int nt, int six, int siz,
+                        const int* rix, int nrec, int riz, int k, float* rec_out,
+                        float* snaps_out, float* illum_out) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  TSW_TRY(fail_injected());
+  Grid* g = reinterpret_cast<Grid*>(h);
+  TSW_ARG(g && src, "null argument");
+  TSW_TRY(ela_shot_checks(g, nt, nrec, riz, k));
+  TSW_ARG(six >= 0 && six < g->nx && siz >= 0 && siz < g->nz, "source outside grid");
+  TSW_ARG(nrec == 0 || (rix && rec_out), "receivers need rix and rec_out");
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  hipStream_t st = ctx->stream;
+  const long long n = g->cells();
+  TSW_TRY(ela_upload(g, 0, lam, mu, bx, bz, st));
+  TSW_TRY(ela_recv(g, rix, nrec, st));
+  TSW_TRY(g->src.ensure((size_t)nt * sizeof(float)));
+  TSW_HIP(hipMemcpyAsync(g->src.p, src, (size_t)nt * sizeof(float), hipMemcpyHostToDevice, st));
+  TSW_TRY(g->rec.ensure((size_t)nt * (nrec > 0 ? nrec : 1) * sizeof(float)));
+  const long long nsnap = (nt - 1) / k + 1;
+  const bool side = snaps_out || illum_out;
+  if (side) {
+    TSW_TRY(g->snaps.ensure((size_t)nsnap * n * sizeof(float)));
+    TSW_TRY(g->illum.ensure((size_t)n * sizeof(float)));
+    launch_fill(g->illum.f(), n, 0.0f, st);
+  }
+  TSW_TRY(ela_forward(g, 0, g->src.f(), nt, six, siz, nrec, riz, k, nrec ? g->rec.f() : nullptr,
+                      side ? g->snaps.f() : nullptr, side ? g->illum.f() : nullptr, st));
+  if (nrec)
+    TSW_HIP(hipMemcpyAsync(rec_out, g->rec.p, (size_t)nt * nrec * sizeof(float),
+                           hipMemcpyDeviceToHost, st));
+  if (snaps_out)
+    TSW_HIP(hipMemcpyAsync(snaps_out, g->snaps.p, (size_t)nsnap * n * sizeof(float),
+                           hipMemcpyDeviceToHost, st));
+  if (illum_out)
+    TSW_HIP(hipMemcpyAsync(illum_out, g->illum.p, (size_t)n * sizeof(float),
+                           hipMemcpyDeviceToHost, st));
+  TSW_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+int tsw_backward_correlate_elastic(tsw_grid_t h, const float* lam, const float* mu,
+                                   const float* bx, const float* bz, const float* data, int nt,
+                                   const int* rix, int nrec, int riz, const float* snaps,
+                                   long long nsnap, float* image_inout, int k) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  TSW_TRY(fail_injected());
+  Grid* g = reinterpret_cast<Grid*>(h);
+  TSW_ARG(g && image_inout, "null argument");
+  TSW_TRY(ela_shot_checks(g, nt, nrec, riz, k));
+  TSW_ARG(nrec == 0 || (rix && data), "receivers need rix and data");
+  TSW_ARG(nsnap >= 0 && (nsnap == 0 || snaps), "snapshots missing");
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  hipStream_t st = ctx->stream;
+  const long long n = g->cells();
+  TSW_TRY(ela_upload(g, 1, lam, mu, bx, bz, st));
+  TSW_TRY(ela_recv(g, rix, nrec, st));
+  TSW_TRY(g->data.ensure((size_t)nt * (nrec > 0 ? nrec : 1) * sizeof(float)));
+  if (nrec)
+    TSW_HIP(hipMemcpyAsync(g->data.p, data, (size_t)nt * nrec * sizeof(float),
+                           hipMemcpyHostToDevice, st));
+  long long used = std::min<long long>(nsnap, (nt - 1) / k + 1);
+  if ((nt - 1) / k >= nsnap) used = 0;
+  TSW_TRY(g->isnap.ensure((size_t)(used > 0 ? used : 1) * n * sizeof(float)));
+  if (used)
+    TSW_HIP(hipMemcpyAsync(g->isnap.p, snaps, (size_t)used * n * sizeof(float),
+                           hipMemcpyHostToDevice, st));
+  TSW_TRY(g->image.ensure((size_t)n * sizeof(float)));
+  TSW_HIP(hipMemcpyAsync(g->image.p, image_inout, (size_t)n * sizeof(float),
+                         hipMemcpyHostToDevice, st));
+  TSW_TRY(ela_backward(g, 1, g->data.f(), nt, nrec, riz, g->isnap.f(), used, g->image.f(), k, st));
+  TSW_HIP(hipMemcpyAsync(image_inout, g->image.p, (size_t)n * sizeof(float),
+                         hipMemcpyDeviceToHost, st));
+  TSW_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+int tsw_migrate_shot_elastic(tsw_grid_t h, const float* const* model_fwd,
+                             const float* const* model_bwd, const float* src, int six, int siz,
+                             const float* data, int nt, const int* rix, int nrec, int riz, int k,
+                             float* image_out, float* illum_out) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  TSW_TRY(fail_injected());
+  Grid* g = reinterpret_cast<Grid*>(h);
+  TSW_ARG(g && model_fwd && model_bwd && src && image_out && illum_out, "null argument");
+  TSW_TRY(ela_shot_checks(g, nt, nrec, riz, k));
+  TSW_ARG(six >= 0 && six < g->nx && siz >= 0 && siz < g->nz, "source outside grid");
+  TSW_ARG(nrec == 0 || (rix && data), "receivers need rix and data");
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  hipStream_t st = ctx->stream;
+  const long long n = g->cells();
+  TSW_TRY(ela_upload(g, 0, model_fwd[0], model_fwd[1], model_fwd[2], model_fwd[3], st));
+  TSW_TRY(ela_upload(g, 1, model_bwd[0], model_bwd[1], model_bwd[2], model_bwd[3], st));
+  TSW_TRY(ela_recv(g, rix, nrec, st));
+  TSW_TRY(g->src.ensure((size_t)nt * sizeof(float)));
+  TSW_HIP(hipMemcpyAsync(g->src.p, src, (size_t)nt * sizeof(float), hipMemcpyHostToDevice, st));
+  TSW_TRY(g->data.ensure((size_t)nt * (nrec > 0 ? nrec : 1) * sizeof(float)));
+  if (nrec)
+    TSW_HIP(hipMemcpyAsync(g->data.p, data, (size_t)nt * nrec * sizeof(float),
+                           hipMemcpyHostToDevice, st));
+  const long long nsnap = (nt - 1) / k + 1;
+  TSW_TRY(g->snaps.ensure((size_t)nsnap * n * sizeof(float)));
+  TSW_TRY(g->illum.ensure((size_t)n * sizeof(float)));
+  TSW_TRY(g->image.ensure((size_t)n * sizeof(float)));
+  launch_fill(g->illum.f(), n, 0.0f, st);
+  launch_fill(g->image.f(), n, 0.0f, st);
+  TSW_TRY(ela_forward(g, 0, g->src.f(), nt, six, siz, nrec, riz, k, nullptr, g->snaps.f(),
+                      g->illum.f(), st));
+  launch_scale(g->illum.f(), n, (float)(g->dt * k), st);
+  TSW_TRY(ela_backward(g, 1, g->data.f(), nt, nrec, riz, g->snaps.f(), nsnap, g->image.f(), k,
+                       st));
+  TSW_HIP(hipMemcpyAsync(image_out, g->image.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
+                         st));
+  TSW_HIP(hipMemcpyAsync(illum_out, g->illum.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
+                         st));
+  TSW_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+}  // extern "C"

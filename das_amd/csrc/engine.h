@@ -62,6 +62,7 @@ struct Grid {
   float dxf = 0, dzf = 0, dtf = 0, rdx2 = 0, rdz2 = 0, dt2 = 0, inj = 0;
   DevBuf sponge, table, buf[3], coef[2], vel, roff, ridx, src, data, rec, snaps, illum, image,
       isnap;
+  DevBuf ef[5], epar[2][4];  // elastic state (vx, vz, sxx, szz, sxz) and models
   long long cells() const { return (long long)nx * nz; }
   AcuStep base() const;  // geometry + constants, no buffers bound
 };

@@ -92,6 +92,29 @@ class TSWRTMEngine:
                                          len(rix), riz, k, N.ptr(img), N.ptr(ill)))
         return img, ill
 
+    def migrate_shot_elastic(self, model_fwd, model_bwd, source_ts, source_ix, source_iz,
+                             receiver_data, receiver_ix, receiver_iz=2, snapshot_interval=2):
+        """Elastic counterpart of migrate_shot: forward in ``model_fwd``
+        (lambda, mu, buoyancy_x, buoyancy_z) with vz snapshots in HBM, back-
+        propagation of the strain-rate gather in ``model_bwd`` -> (image, illumination)."""
+        s = self.solver
+        src, six, siz, data, rix, riz, k = self._shot_args(
+            source_ts, source_ix, source_iz, receiver_data, receiver_ix, receiver_iz,
+            snapshot_interval)
+        if len(src) != data.shape[0]:
+            raise ValueError("elastic migration needs source and data of the same length")
+        mf = s._elastic_model(model_fwd)
+        mb = s._elastic_model(model_bwd)
+        PF = (N.P * 4)(*[N.ptr(a) for a in mf])
+        PB = (N.P * 4)(*[N.ptr(a) for a in mb])
+        img = np.empty((s.nz, s.nx), np.float32)
+        ill = np.empty_like(img)
+        s._sync_sponge()
+        N.check(N.lib().tsw_migrate_shot_elastic(s._grid, PF, PB, N.ptr(src), six, siz,
+                                                 N.ptr(data), data.shape[0], N.ptr(rix), len(rix),
+                                                 riz, k, N.ptr(img), N.ptr(ill)))
+        return img, ill
+
     def migrate_multiple_shots(self, flood_models: FloodModels, source_waveforms: List[np.ndarray],
                                source_coords_ix: List[int], source_coords_iz: List[int],
                                receiver_gathers: List[np.ndarray], receiver_ix: np.ndarray,

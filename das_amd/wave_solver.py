@@ -176,3 +176,77 @@ class WaveSolver2D:
                                                len(snaps), N.ptr(img), k))
         if img is not image_out:
             image_out[...] = img
+
+    # ------------------------------------------------------------- elastic
+    # The reference declares elastic_2d_step (wave_solver.py:36-53) but never
+    # drives it; solver_type="elastic" is its only hook.  These methods are the
+    # elastic counterparts of the acoustic loops, with the conventions of
+    # include/tswrtm.h (vertical force source, vertical strain-rate receivers,
+    # vz imaging).
+
+    def _elastic_model(self, model):
+        if len(model) != 4:
+            raise ValueError("elastic model is (lambda, mu, buoyancy_x, buoyancy_z)")
+        arrs = [N.f32(a) for a in model]
+        for a in arrs:
+            if a.shape != (self.nz, self.nx):
+                raise ValueError(f"elastic model array has shape {a.shape}")
+        return arrs
+
+    def forward_propagate_elastic(self, model, source_ts, source_ix, source_iz, receiver_ix,
+                                  receiver_iz=2, save_wavefield=False, snapshot_interval=1,
+                                  return_illumination=False):
+        """Elastic forward modelling -> (strain-rate traces (nt, n_rec),
+        vz snapshots or None[, sum of squared snapshots])."""
+        par = self._elastic_model(model)
+        src = N.f32(np.asarray(source_ts).reshape(-1))
+        nt = len(src)
+        six = _norm_index(source_ix, self.nx, "source_ix")
+        siz = _norm_index(source_iz, self.nz, "source_iz")
+        rix = _norm_indices(receiver_ix, self.nx, "receiver_ix")
+        riz = _norm_index(receiver_iz, self.nz, "receiver_iz")
+        k = int(snapshot_interval)
+        if k < 1 or nt == 0:
+            raise ValueError("need snapshot_interval >= 1 and a non-empty source")
+        rec = np.zeros((nt, len(rix)), np.float32)
+        snaps = (np.empty(((nt - 1) // k + 1, self.nz, self.nx), np.float32)
+                 if save_wavefield else None)
+        ill = np.empty((self.nz, self.nx), np.float32) if return_illumination else None
+        self._sync_sponge()
+        N.check(N.lib().tsw_forward_elastic(self._grid, *[N.ptr(a) for a in par], N.ptr(src), nt,
+                                            six, siz, N.ptr(rix), len(rix), riz, k, N.ptr(rec),
+                                            N.ptr(snaps), N.ptr(ill)))
+        return (rec, snaps, ill) if return_illumination else (rec, snaps)
+
+    def backward_propagate_and_correlate_elastic(self, model, receiver_data, receiver_ix,
+                                                 receiver_iz, forward_wavefields, image_out,
+                                                 snapshot_interval=1):
+        """Elastic back-propagation of the data + vz imaging into image_out."""
+        par = self._elastic_model(model)
+        data = N.f32(receiver_data)
+        nt, n_rec = data.shape
+        rix = _norm_indices(receiver_ix, self.nx, "receiver_ix")
+        if len(rix) != n_rec:
+            raise ValueError(f"{len(rix)} receiver positions for {n_rec} traces")
+        riz = _norm_index(receiver_iz, self.nz, "receiver_iz")
+        k = int(snapshot_interval)
+        snaps = N.f32(forward_wavefields)
+        img = N.f32(image_out)
+        self._sync_sponge()
+        N.check(N.lib().tsw_backward_correlate_elastic(
+            self._grid, *[N.ptr(a) for a in par], N.ptr(data), nt, N.ptr(rix), n_rec, riz,
+            N.ptr(snaps), len(snaps), N.ptr(img), k))
+        if img is not image_out:
+            image_out[...] = img
+
+
+def elastic_parameters(vp, vs, rho):
+    """(lambda, mu, buoyancy_x, buoyancy_z) float32 from Vp, Vs and density,
+    with cell-centred buoyancy 1/rho on both staggered velocity grids."""
+    vp = np.asarray(vp, np.float64)
+    vs = np.asarray(vs, np.float64)
+    rho = np.asarray(rho, np.float64)
+    mu = rho * vs * vs
+    lam = rho * vp * vp - 2.0 * mu
+    b = 1.0 / rho
+    return tuple(np.ascontiguousarray(a, np.float32) for a in (lam, mu, b, b))

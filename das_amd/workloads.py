@@ -101,6 +101,64 @@ def make_data_config0(cfg: Config0, solver):
     return bandpass(rec, cfg.dt)
 
 
+def layered_faulted_vp(nz, nx, dx, seed=0, v_min=1800.0, v_max=3500.0, n_layers=5,
+                       fault_x_m=None, dip_deg=60.0, throw_m=50.0, perturb=0.03, corr_m=25.0):
+    """Faulted layered Vp model (v_min..v_max with depth) with a smoothed
+    seeded +-3% perturbation."""
+    z = (np.arange(nz) * dx)[:, None]
+    x = (np.arange(nx) * dx)[None, :]
+    depth = nz * dx
+    fault_x_m = 0.5 * nx * dx if fault_x_m is None else fault_x_m
+    xf = fault_x_m + z / np.tan(np.deg2rad(dip_deg))
+    shift = np.where(x > xf, 0.0, -throw_m)
+    layer = np.clip(np.floor((z - shift) / (depth / n_layers)), 0, n_layers - 1)
+    v = v_min + (v_max - v_min) * layer / (n_layers - 1)
+    rng = np.random.default_rng(seed)
+    noise = ndimage.gaussian_filter(rng.standard_normal((nz, nx)), corr_m / dx)
+    noise /= max(noise.std(), 1e-12)
+    return (v * (1.0 + np.clip(0.01 * noise, -perturb, perturb))).astype(np.float32)
+
+
+def elastic_from_vp(vp):
+    """Vs = Vp/1.8, Gardner density rho = 310 Vp^0.25 (SI), then
+    (lambda, mu, buoyancy_x, buoyancy_z)."""
+    from .wave_solver import elastic_parameters
+
+    vp = np.asarray(vp, np.float64)
+    return elastic_parameters(vp, vp / 1.8, 310.0 * vp ** 0.25)
+
+
+@dataclass
+class Config1:
+    """Plain 2D elastic P-SV RTM, one shot: 400 x 1600 at 5 m, dt 0.4 ms,
+    nt 8000; vertical-force Ricker 2 Hz; 800 vertical strain-rate receivers
+    at 10 m spacing."""
+    nz: int = 400
+    nx: int = 1600
+    dx: float = 5.0
+    dt: float = 4e-4
+    nt: int = 8000
+    n_sponge: int = 30
+    k: int = 2
+    f0: float = 2.0
+    n_rec: int = 800
+    seed: int = 1
+    siz: int = 2
+    riz: int = 2
+
+    def __post_init__(self):
+        vp = layered_faulted_vp(self.nz, self.nx, self.dx, self.seed)
+        self.model_true = elastic_from_vp(vp)
+        self.model_mig = elastic_from_vp(smooth(vp, 100.0, self.dx))
+        self.src, _ = ricker_wavelet(self.f0, self.dt, self.nt * self.dt)
+        self.six = self.nx // 2
+        self.rix = (np.arange(self.n_rec) * 2).astype(np.int32)
+
+    @property
+    def cell_updates(self):
+        return 2 * self.nt * self.nx * self.nz
+
+
 def correlogram_gather(rng, nt, dt, offsets_m, v_app=1800.0, f0=2.5, f_lo=0.8, f_hi=4.5):
     """Whitened, 1-bit-normalised synthetic cross-correlograms (nt, nrec):
     band-limited (f_lo..f_hi) noise correlated against a Ricker arrival delayed

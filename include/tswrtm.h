@@ -133,6 +133,32 @@ int tsw_migrate_shots(tsw_grid_t grid, const float* vel_fwd, const float* vel_bw
  * 'symm') with the reference's float32 kernel. */
 int tsw_laplacian(const float* image, float* out, int nx, int nz, double dx, double dz);
 
+/* Elastic P-SV driver around elastic_2d_step (wave2d.c:95-212), which has no
+ * caller in the reference (wave_solver.py:36-53 only declares it).  Build-
+ * defined conventions: iteration it adds the vertical force (dt*bz)*s[it] to
+ * vz at the source, snapshots / images the post-injection vz, steps, then
+ * records the vertical strain rate (4th-order staggered d(vz)/dz, rows outside
+ * the grid read 0) at the receivers on row riz.  Back-propagation injects the
+ * data like the source and images vz against the forward vz snapshots with the
+ * acoustic loop's snapshot cursor.  Models: lambda, mu, buoyancy_x,
+ * buoyancy_z, each (nz, nx).  rec_out: (nt, nrec); snaps_out: NULL or
+ * ((nt-1)/k+1, nz, nx) of vz; illum_out: NULL or sum of squared snapshots. */
+int tsw_forward_elastic(tsw_grid_t grid, const float* lam, const float* mu, const float* bx,
+                        const float* bz, const float* src, int nt, int six, int siz,
+                        const int* rix, int nrec, int riz, int k, float* rec_out,
+                        float* snaps_out, float* illum_out);
+int tsw_backward_correlate_elastic(tsw_grid_t grid, const float* lam, const float* mu,
+                                   const float* bx, const float* bz, const float* data, int nt,
+                                   const int* rix, int nrec, int riz, const float* snaps,
+                                   long long nsnap, float* image_inout, int k);
+/* Elastic RTM of one shot: model_fwd / model_bwd are 4 pointers each
+ * (lambda, mu, buoyancy_x, buoyancy_z); illumination = sum of squared vz
+ * snapshots * dt*k. */
+int tsw_migrate_shot_elastic(tsw_grid_t grid, const float* const* model_fwd,
+                             const float* const* model_bwd, const float* src, int six, int siz,
+                             const float* data, int nt, const int* rix, int nrec, int riz, int k,
+                             float* image_out, float* illum_out);
+
 /* Batched acoustic RTM (no reference counterpart; it batches what
  * TSWRTMEngine.migrate_multiple_shots, rtm_engine.py:125-143, runs one shot at
  * a time).  Uploads every shot's models, source and data, sizes the HBM state

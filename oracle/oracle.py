@@ -59,6 +59,11 @@ def lib():
             [_P, _P, _P, _I, _I, _D, _D, _D, _I, _P, _I, _I, _P, _I, _P, _I, _I, _I, _P, _P])
         L.ora_migrate_shot_f64.restype = _I
         L.ora_laplacian_f32.argtypes = [_P, _P, _I, _I, _D, _D]
+        for sfx in ("f32", "f64"):
+            getattr(L, f"ora_elastic_forward_{sfx}").argtypes = (
+                [_P] * 5 + [_I, _I, _D, _D, _D, _I, _P, _I, _I, _I, _P, _I, _I, _I, _P, _P, _P])
+            getattr(L, f"ora_elastic_backward_{sfx}").argtypes = (
+                [_P] * 5 + [_I, _I, _D, _D, _D, _I, _P, _I, _P, _I, _I, _P, _L, _P, _I])
         _lib = L
     return _lib
 
@@ -254,3 +259,62 @@ def ricker(f0, dt, t_length):
     tau = t - 1.0 / f0
     a = (np.pi * f0 * tau) ** 2
     return ((1.0 - 2.0 * a) * np.exp(-a)).astype(np.float32), t
+
+
+# ---------------------------------------------------------------- elastic
+def elastic_parameters(vp, vs, rho):
+    """(lambda, mu, buoyancy_x, buoyancy_z) from Vp, Vs, rho (cell-centred)."""
+    vp = np.asarray(vp, np.float64)
+    vs = np.asarray(vs, np.float64)
+    rho = np.asarray(rho, np.float64)
+    mu = rho * vs * vs
+    lam = rho * vp * vp - 2.0 * mu
+    b = 1.0 / rho
+    return tuple(np.ascontiguousarray(a, np.float32) for a in (lam, mu, b, b))
+
+
+def elastic_forward(g: Grid, model, src, six, siz, rix, riz=2, k=1, save=False, illum=False,
+                    dtype=np.float32):
+    """Elastic forward loop -> (rec (nt, nrec) strain rate, vz snaps or None,
+    sum of squared vz snapshots or None)."""
+    T = np.dtype(dtype)
+    par = [_c(a, T) for a in model]
+    src = _c(src, T)
+    nt = len(src)
+    rix = _c(rix, np.int32)
+    rec = np.zeros((nt, len(rix)), dtype=T)
+    ns = (nt - 1) // k + 1
+    snaps = np.zeros((ns, g.nz, g.nx), dtype=T) if save else None
+    ill = np.zeros((g.nz, g.nx), dtype=T) if illum else None
+    sp = g.sponge(T)
+    getattr(lib(), f"ora_elastic_forward_{_sfx(T)}")(
+        *[_ptr(a) for a in par], _ptr(sp), g.nx, g.nz, g.dx, g.dz, g.dt, int(g.free_surface),
+        _ptr(src), nt, int(six), int(siz), _ptr(rix), len(rix), int(riz), int(k), _ptr(rec),
+        _ptr(snaps), _ptr(ill))
+    return rec, snaps, ill
+
+
+def elastic_backward(g: Grid, model, data, rix, riz, snaps, image, k=1):
+    T = image.dtype
+    par = [_c(a, T) for a in model]
+    data = _c(data, T)
+    nt, nrec = data.shape
+    rix = _c(rix, np.int32)
+    snaps = _c(snaps, T)
+    sp = g.sponge(T)
+    getattr(lib(), f"ora_elastic_backward_{_sfx(T)}")(
+        *[_ptr(a) for a in par], _ptr(sp), g.nx, g.nz, g.dx, g.dz, g.dt, int(g.free_surface),
+        _ptr(data), nt, _ptr(rix), nrec, int(riz), _ptr(snaps), len(snaps), _ptr(image), int(k))
+    return image
+
+
+def elastic_migrate_shot(g: Grid, model_fwd, model_bwd, src, six, siz, data, rix, riz=2, k=2,
+                         dtype=np.float32):
+    """Elastic RTM of one shot -> (image, illumination = sum vz_snap^2 * dt*k)."""
+    T = np.dtype(dtype)
+    _, snaps, ill = elastic_forward(g, model_fwd, src, six, siz, rix, riz, k, save=True,
+                                    illum=True, dtype=T)
+    ill = ill * T.type(g.dt * k)
+    img = np.zeros((g.nz, g.nx), dtype=T)
+    elastic_backward(g, model_bwd, data, rix, riz, snaps, img, k)
+    return img, ill

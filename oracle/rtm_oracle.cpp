@@ -262,6 +262,76 @@ void backward_loop(const Stepper<T>& st, const T* vel, const T* sponge, int nx, 
   }
 }
 
+// Elastic driver conventions (no reference caller exists for
+// elastic_2d_step, wave2d.c:95-212; wave_solver.py:36-53 only declares it):
+//   iteration it: vz[src] += (dt*bz[src]) * s[it]  (vertical force), side work
+//   on the post-injection vz (snapshot / illumination, or imaging), one
+//   elastic step, then the receivers record the vertical strain rate
+//   D-z vz = (c1*(vz[i]-vz[i-nx]) + c2*(vz[i+nx]-vz[i-2nx]))/dz at (riz, rix)
+//   with rows outside the grid reading 0.  Back-propagation injects the data
+//   like the source, receiver by receiver, and images vz against the forward
+//   vz snapshots with the acoustic loop's snapshot cursor.
+template <typename T>
+T strain_rate(const T* vz, int nx, int nz, int iz, int ix, T rdz) {
+  auto at = [&](int z) -> T { return (z < 0 || z >= nz) ? T(0) : vz[(size_t)z * nx + ix]; };
+  const T a = stg_c1<T>(), b = stg_c2<T>();
+  return (a * (at(iz) - at(iz - 1)) + b * (at(iz + 1) - at(iz - 2))) * rdz;
+}
+
+template <typename T>
+struct ElaModel {
+  const T *lam, *mu, *bx, *bz, *sponge;
+};
+
+template <typename T>
+void elastic_forward_loop(const ElaModel<T>& m, int nx, int nz, T dx, T dz, T dt, int fs,
+                          const T* src, int nt, int six, int siz, const int* rix, int nrec,
+                          int riz, int k, T* rec_out, T* snaps_out, T* illum) {
+  const size_t n = (size_t)nx * nz;
+  std::vector<T> f[5];
+  for (auto& v : f) v.assign(n, T(0));
+  const size_t sc = (size_t)siz * nx + six;
+  const T rdz = T(1) / dz;
+  size_t snap = 0;
+  for (int it = 0; it < nt; ++it) {
+    f[1][sc] = f[1][sc] + (dt * m.bz[sc]) * src[it];
+    if (it % k == 0) {
+      if (snaps_out) std::memcpy(snaps_out + snap * n, f[1].data(), n * sizeof(T));
+      if (illum)
+        for (size_t i = 0; i < n; ++i) illum[i] = illum[i] + f[1][i] * f[1][i];
+      ++snap;
+    }
+    elastic_step<T>(f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data(), m.lam, m.mu,
+                    m.bx, m.bz, m.sponge, nx, nz, dx, dz, dt, fs);
+    if (rec_out)
+      for (int r = 0; r < nrec; ++r)
+        rec_out[(size_t)it * nrec + r] = strain_rate<T>(f[1].data(), nx, nz, riz, rix[r], rdz);
+  }
+}
+
+template <typename T>
+void elastic_backward_loop(const ElaModel<T>& m, int nx, int nz, T dx, T dz, T dt, int fs,
+                           const T* data, int nt, const int* rix, int nrec, int riz,
+                           const T* snaps, long long nsnap, T* image, int k, T w) {
+  const size_t n = (size_t)nx * nz;
+  std::vector<T> f[5];
+  for (auto& v : f) v.assign(n, T(0));
+  long long cursor = (nt - 1) / k;
+  for (int it = nt - 1; it >= 0; --it) {
+    for (int r = 0; r < nrec; ++r) {
+      const size_t c = (size_t)riz * nx + rix[r];
+      f[1][c] = f[1][c] + (dt * m.bz[c]) * data[(size_t)it * nrec + r];
+    }
+    if (it % k == 0 && cursor >= 0 && cursor < nsnap) {
+      const T* s = snaps + (size_t)cursor * n;
+      for (size_t i = 0; i < n; ++i) image[i] = image[i] + (s[i] * f[1][i]) * w;
+      --cursor;
+    }
+    elastic_step<T>(f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data(), m.lam, m.mu,
+                    m.bx, m.bz, m.sponge, nx, nz, dx, dz, dt, fs);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -380,6 +450,26 @@ int ora_migrate_shot_f64(const double* vel_fwd, const double* vel_bwd, const dou
                         snaps.data(), nsnap, image, k);
   return 0;
 }
+
+#define ORA_ELA(T, SFX)                                                                           \
+  void ora_elastic_forward_##SFX(const T* lam, const T* mu, const T* bx, const T* bz,             \
+                                 const T* sp, int nx, int nz, double dx, double dz, double dt,    \
+                                 int fs, const T* src, int nt, int six, int siz, const int* rix,  \
+                                 int nrec, int riz, int k, T* rec_out, T* snaps_out, T* illum) {  \
+    ElaModel<T> m{lam, mu, bx, bz, sp};                                                           \
+    elastic_forward_loop<T>(m, nx, nz, T(dx), T(dz), T(dt), fs, src, nt, six, siz, rix, nrec,     \
+                            riz, k, rec_out, snaps_out, illum);                                   \
+  }                                                                                               \
+  void ora_elastic_backward_##SFX(const T* lam, const T* mu, const T* bx, const T* bz,            \
+                                  const T* sp, int nx, int nz, double dx, double dz, double dt,   \
+                                  int fs, const T* data, int nt, const int* rix, int nrec,        \
+                                  int riz, const T* snaps, long long nsnap, T* image, int k) {    \
+    ElaModel<T> m{lam, mu, bx, bz, sp};                                                           \
+    elastic_backward_loop<T>(m, nx, nz, T(dx), T(dz), T(dt), fs, data, nt, rix, nrec, riz,       \
+                             snaps, nsnap, image, k, T(dt * k));                                  \
+  }
+ORA_ELA(float, f32)
+ORA_ELA(double, f64)
 
 // 5-point Laplacian with symmetric (edge-duplicating) boundary, the action of
 // scipy.signal.convolve2d(img, K, 'same', 'symm') for the kernel of

@@ -303,6 +303,91 @@ def gen_migrate(rng):
     np.savez_compressed(os.path.join(HERE, "migrate.npz"), **out)
 
 
+def gen_elastic_loops(rng):
+    """Elastic driver loops (build-defined conventions: vertical force
+    vz += (dt*bz)*s, vertical strain-rate receivers, vz imaging) around the
+    reference's elastic_2d_step compiled from source, on padded buffers."""
+    f32 = np.float32
+    c1, c2 = f32(9) / f32(8), f32(-1) / f32(24)
+    out = {}
+    cases = [(40, 30, 5.0, 4e-4, 5, 1, 50, 2, 20, 2, [0, 3, 3, 10, 20, 39], 2),
+             (33, 26, 4.0, 3e-4, 4, 0, 41, 3, 5, 3, [2, 7, 30], 4),
+             (48, 36, 5.0, 4e-4, 6, 1, 60, 1, 24, 0, [1, 24, 46], 1)]
+    for i, (nx, nz, dx, dt, nsp, fs, nt, k, six, siz, rix, riz) in enumerate(cases):
+        sponge = ref_sponge(nx, nz, nsp, 0.02, fs)
+        vp = (2000 + 800 * rng.random((nz, nx))).astype(f32)
+        rho = (310.0 * vp.astype(np.float64) ** 0.25)
+        vs = vp / 1.8
+        mu = (rho * vs * vs).astype(f32)
+        lam = (rho * vp * vp - 2 * rho * vs * vs).astype(f32)
+        b = (1.0 / rho).astype(f32)
+        par = [np.ascontiguousarray(a) for a in (lam, mu, b, b)]
+        src = (rng.standard_normal(nt) * 1e3).astype(f32)
+        rix = np.array(rix)
+        dtf, rdz = f32(dt), f32(1) / f32(dx)
+
+        def run(inject, it_order, side, record):
+            pad = [np.zeros((nz + 2, nx), f32) for _ in range(5)]
+            vz = pad[1][2:]
+            for it in it_order:
+                inject(vz, it)
+                side(vz, it)
+                R.elastic_2d_step(*[p.ctypes.data + 2 * nx * 4 for p in pad],
+                                  *[ptr(a) for a in par], ptr(sponge), nx, nz, dx, dx, dt, fs)
+                record(vz, it)
+
+        def strain(vz, iz, ix):
+            def at(z):
+                return vz[z, ix] if 0 <= z < nz else f32(0)
+            return (c1 * (at(iz) - at(iz - 1)) + c2 * (at(iz + 1) - at(iz - 2))) * rdz
+
+        rec = np.zeros((nt, len(rix)), f32)
+        snaps = []
+        illum = np.zeros((nz, nx), f32)
+
+        def inj_src(vz, it):
+            vz[siz, six] = vz[siz, six] + (dtf * par[3][siz, six]) * src[it]
+
+        def side_fwd(vz, it):
+            if it % k == 0:
+                snaps.append(vz.copy())
+                illum[...] = illum + vz * vz
+
+        def rec_fwd(vz, it):
+            for r in range(len(rix)):
+                rec[it, r] = strain(vz, riz, rix[r])
+
+        run(inj_src, range(nt), side_fwd, rec_fwd)
+        snaps = np.array(snaps)
+        data = (rec + f32(1e-9) * rng.standard_normal(rec.shape).astype(f32)).astype(f32)
+        image = np.zeros((nz, nx), f32)
+        w = f32(dt * k)
+        cursor = [(nt - 1) // k]
+
+        def inj_data(vz, it):
+            for r in range(len(rix)):
+                vz[riz, rix[r]] = vz[riz, rix[r]] + (dtf * par[3][riz, rix[r]]) * data[it, r]
+
+        def side_bwd(vz, it):
+            if it % k == 0 and 0 <= cursor[0] < len(snaps):
+                image[...] = image + (snaps[cursor[0]] * vz) * w
+                cursor[0] -= 1
+
+        run(inj_data, range(nt - 1, -1, -1), side_bwd, lambda vz, it: None)
+        out[f"c{i}_ints"] = np.array([nx, nz, nsp, fs, nt, k, six, siz, riz])
+        out[f"c{i}_flt"] = np.array([dx, dt, 0.02])
+        out[f"c{i}_par"] = np.stack(par)
+        out[f"c{i}_src"] = src
+        out[f"c{i}_rix"] = rix.astype(np.int32)
+        out[f"c{i}_rec"] = rec
+        out[f"c{i}_snaps"] = snaps
+        out[f"c{i}_illum"] = illum
+        out[f"c{i}_data"] = data
+        out[f"c{i}_image"] = image
+    out["ncase"] = np.array([len(cases)])
+    np.savez_compressed(os.path.join(HERE, "elastic_loops.npz"), **out)
+
+
 def main():
     rng = np.random.default_rng(20251203)
     gen_sponge()
@@ -311,6 +396,7 @@ def main():
     gen_correlate(rng)
     gen_loops(rng)
     gen_migrate(rng)
+    gen_elastic_loops(np.random.default_rng(20251204))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -242,3 +242,42 @@ def test_ricker_closed_form():
     iz = int(np.argmin(np.abs(t - tz)))
     assert abs(w[iz]) < 5e-3
     assert np.argmax(w) == i0
+
+
+def _ela_case(g, i):
+    nx, nz, nsp, fs, nt, k, six, siz, riz = (int(v) for v in g[f"c{i}_ints"])
+    dx, dt, alpha = (float(v) for v in g[f"c{i}_flt"])
+    grid = ora.Grid(nx, nz, dx, dx, dt, nsp, alpha, bool(fs))
+    return grid, nt, k, six, siz, riz
+
+
+def test_elastic_loops_golden(golden):
+    g = golden("elastic_loops")
+    for i in range(int(g["ncase"][0])):
+        grid, nt, k, six, siz, riz = _ela_case(g, i)
+        model = tuple(g[f"c{i}_par"])
+        rec, snaps, ill = ora.elastic_forward(grid, model, g[f"c{i}_src"], six, siz,
+                                              g[f"c{i}_rix"], riz, k, save=True, illum=True)
+        np.testing.assert_array_equal(rec, g[f"c{i}_rec"], err_msg=f"rec {i}")
+        np.testing.assert_array_equal(snaps, g[f"c{i}_snaps"], err_msg=f"snaps {i}")
+        np.testing.assert_array_equal(ill, g[f"c{i}_illum"], err_msg=f"illum {i}")
+        img = np.zeros((grid.nz, grid.nx), np.float32)
+        ora.elastic_backward(grid, model, g[f"c{i}_data"], g[f"c{i}_rix"], riz, snaps, img, k)
+        np.testing.assert_array_equal(img, g[f"c{i}_image"], err_msg=f"image {i}")
+
+
+def test_elastic_linearity_and_fp64():
+    nz, nx, dt = 30, 40, 4e-4
+    grid = ora.Grid(nx, nz, 5.0, 5.0, dt, 6)
+    vp = np.full((nz, nx), 2500.0)
+    model = ora.elastic_parameters(vp, vp / 1.8, 310.0 * vp ** 0.25)
+    src, _ = ora.ricker(25.0, dt, 0.04)
+    rix = np.arange(2, nx - 2, 3)
+    r1, _, _ = ora.elastic_forward(grid, model, src, 20, 2, rix, 2)
+    r2, _, _ = ora.elastic_forward(grid, model, 2 * src, 20, 2, rix, 2)
+    np.testing.assert_allclose(r2, 2 * r1, rtol=0, atol=1e-30)
+    r64, _, _ = ora.elastic_forward(grid, model, src, 20, 2, rix, 2, dtype=np.float64)
+    assert _rel(r1, r64) < 1e-4
+    z = np.zeros_like(src)
+    r0, _, _ = ora.elastic_forward(grid, model, z, 20, 2, rix, 2)
+    assert not r0.any()
