@@ -56,6 +56,50 @@ def cpu_baseline(cfg, budget_cell_updates):
                        "and snapshots in host RAM as the reference")}
 
 
+ELASTIC_BYTES_PER_CELL_UPDATE = 60.0  # 5 fields read + written, 5 parameters read
+
+
+def elastic_leg(args, log):
+    """Plain 2D elastic P-SV RTM of one shot (400 x 1600, nt 8000, dt 0.4 ms,
+    800 strain-rate receivers): data modelled in the true model, then the
+    migration timed as a resident one-shot batch (hipGraph-captured steps)."""
+    import torch
+
+    from das_amd.batch import ShotBatch
+    from das_amd.wave_solver import WaveSolver2D
+    from das_amd.workloads import Config1, ElasticShot
+
+    cfg = Config1()
+    solver = WaveSolver2D(cfg.nx, cfg.nz, cfg.dx, cfg.dx, cfg.dt, n_sponge=cfg.n_sponge,
+                          solver_type="elastic")
+    t0 = time.time()
+    data, _ = solver.forward_propagate_elastic(cfg.model_true, cfg.src, cfg.six, cfg.siz,
+                                               cfg.rix, cfg.riz)
+    log(f"elastic data modelled in {time.time() - t0:.1f}s")
+    shot = ElasticShot(nx=cfg.nx, nz=cfg.nz, nt=cfg.nt, model_fwd=cfg.model_mig,
+                       model_bwd=cfg.model_mig, src=cfg.src, six=cfg.six, siz=cfg.siz,
+                       data=data, rix=cfg.rix, riz=cfg.riz)
+    b = ShotBatch([shot], cfg.dx, cfg.dx, cfg.dt, n_sponge=cfg.n_sponge, k=cfg.k)
+    b.run("batched")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run("batched")
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms, nl = b.time_kernels("elastic")
+    b.close()
+    cu = cfg.cell_updates
+    ach = ELASTIC_BYTES_PER_CELL_UPDATE * cu / (ms * 1e-3) / 1e9
+    return {"workload": "elastic P-SV RTM, 1 shot, 400x1600, nt=8000, dt=0.4 ms, 800 receivers",
+            "value": round(cu / el / 1e9, 3), "unit": "Gcell-updates/s",
+            "ms_per_shot": round(el * 1e3, 2),
+            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "kernel": "k_ela_batch (velocity + stress passes)", "launches": nl,
+                         "avg_launch_us": round(ms * 1e3 / max(nl, 1), 3),
+                         "algorithmic_bytes_per_cell_update": ELASTIC_BYTES_PER_CELL_UPDATE}}
+
+
 def traffic_from_profiles():
     """HBM bytes per launch of the step kernel from the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
@@ -79,6 +123,7 @@ def main():
                          "blocked config c")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-elastic", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=2.5, help="G cell-updates")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -163,6 +208,8 @@ def main():
                             "mode": "one shot at a time, one launch per time step"}
         out["speedup_vs_plain"] = round(shots_per_s / (cfg.n_shots / tp), 2)
     batch.close()
+    if not args.no_elastic:
+        out["elastic"] = elastic_leg(args, log)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget * 1e9)
     print(json.dumps(out), flush=True)

@@ -49,11 +49,11 @@ SIGNATURES = {
     "tsw_kernel_configs": (I, []),
     "tsw_kernel_config": (I, [I] + [ctypes.POINTER(I)] * 4),
     "tsw_batch_run": (I, [P, I]),
-    "tsw_batch_time_kernels": (I, [P, ctypes.POINTER(D), ctypes.POINTER(I)]),
+    "tsw_batch_time_kernels": (I, [P, I, ctypes.POINTER(D), ctypes.POINTER(I)]),
     "tsw_batch_shot_result": (I, [P, I, P, P]),
     "tsw_batch_stack": (I, [P, I, I, P, P]),
     "tsw_batch_info": (I, [P, ctypes.POINTER(LL), ctypes.POINTER(LL), ctypes.POINTER(I),
-                           ctypes.POINTER(I)]),
+                           ctypes.POINTER(I), ctypes.POINTER(LL)]),
     "tsw_batch_destroy": (I, [P]),
 }
 
@@ -65,7 +65,11 @@ class ShotDesc(ctypes.Structure):
     """tsw_shot_desc of include/tswrtm.h."""
     _fields_ = [("nx", I), ("nz", I), ("nt", I), ("vel_fwd", P), ("vel_bwd", P), ("src", P),
                 ("six", I), ("siz", I), ("data", P), ("rix", P), ("nrec", I), ("riz", I),
-                ("x0", I), ("z0", I)]
+                ("x0", I), ("z0", I), ("kind", I), ("ela_fwd", P * 4), ("ela_bwd", P * 4)]
+
+
+SHOT_ACOUSTIC = 0
+SHOT_ELASTIC = 1
 
 
 class TswError(RuntimeError):

@@ -26,7 +26,8 @@ def kernel_configs():
 
 
 class ShotBatch:
-    """A resident batch of shots (objects with the fields of workloads.Shot)."""
+    """A resident batch of shots: objects with the fields of workloads.Shot
+    (acoustic) or workloads.ElasticShot (kind == "elastic")."""
 
     def __init__(self, shots: Sequence, dx: float, dz: float, dt: float, n_sponge: int = 30,
                  sponge_alpha: float = 0.0015, free_surface_top: bool = True, k: int = 2,
@@ -36,17 +37,31 @@ class ShotBatch:
         keep = []
         descs = (N.ShotDesc * len(self.shots))()
         for d, s in zip(descs, self.shots):
-            vf, vb = N.f32(s.vel_fwd), N.f32(s.vel_bwd)
             src = N.f32(np.asarray(s.src).reshape(-1))
             data = N.f32(s.data)
             rix = N.i32(s.rix)
-            if vf.shape != (s.nz, s.nx) or vb.shape != (s.nz, s.nx):
-                raise ValueError("model shape does not match the shot grid")
             if len(src) != s.nt or data.shape != (s.nt, len(rix)):
                 raise ValueError("source / data length must equal the shot's nt")
-            keep += [vf, vb, src, data, rix]
+            keep += [src, data, rix]
+            if getattr(s, "kind", "acoustic") == "elastic":
+                mf = [N.f32(a) for a in s.model_fwd]
+                mb = [N.f32(a) for a in s.model_bwd]
+                if any(a.shape != (s.nz, s.nx) for a in mf + mb) or len(mf) != 4 or len(mb) != 4:
+                    raise ValueError("elastic model shape does not match the shot grid")
+                keep += mf + mb
+                d.kind = N.SHOT_ELASTIC
+                for j in range(4):
+                    d.ela_fwd[j] = N.ptr(mf[j])
+                    d.ela_bwd[j] = N.ptr(mb[j])
+            else:
+                vf, vb = N.f32(s.vel_fwd), N.f32(s.vel_bwd)
+                if vf.shape != (s.nz, s.nx) or vb.shape != (s.nz, s.nx):
+                    raise ValueError("model shape does not match the shot grid")
+                keep += [vf, vb]
+                d.kind = N.SHOT_ACOUSTIC
+                d.vel_fwd, d.vel_bwd = N.ptr(vf), N.ptr(vb)
             d.nx, d.nz, d.nt = s.nx, s.nz, s.nt
-            d.vel_fwd, d.vel_bwd, d.src = N.ptr(vf), N.ptr(vb), N.ptr(src)
+            d.src = N.ptr(src)
             d.six, d.siz = int(s.six), int(s.siz)
             d.data, d.rix, d.nrec, d.riz = N.ptr(data), N.ptr(rix), len(rix), int(s.riz)
             d.x0, d.z0 = int(getattr(s, "x0", 0)), int(getattr(s, "z0", 0))
@@ -70,11 +85,12 @@ class ShotBatch:
         m = {"batched": N.RUN_BATCHED, "plain": N.RUN_PLAIN}[mode]
         N.check(N.lib().tsw_batch_run(self._h, m))
 
-    def time_kernels(self):
-        """One run with a HIP event pair around every step launch ->
+    def time_kernels(self, lane: str = "acoustic"):
+        """One run of a lane with a HIP event pair around every launch ->
         (summed kernel milliseconds, launches)."""
         ms, n = N.D(), N.I()
-        N.check(N.lib().tsw_batch_time_kernels(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        N.check(N.lib().tsw_batch_time_kernels(self._h, 1 if lane == "elastic" else 0,
+                                               ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
     def shot_result(self, i: int):
@@ -91,9 +107,10 @@ class ShotBatch:
         return img, ill
 
     def info(self) -> dict:
-        cu, nb = N.LL(), N.LL()
+        cu, nb, ecu = N.LL(), N.LL(), N.LL()
         steps, graphs = N.I(), N.I()
         N.check(N.lib().tsw_batch_info(self._h, ctypes.byref(cu), ctypes.byref(nb),
-                                       ctypes.byref(steps), ctypes.byref(graphs)))
+                                       ctypes.byref(steps), ctypes.byref(graphs),
+                                       ctypes.byref(ecu)))
         return {"cell_updates": cu.value, "device_bytes": nb.value, "steps": steps.value,
-                "graphs": graphs.value}
+                "graphs": graphs.value, "elastic_cell_updates": ecu.value}

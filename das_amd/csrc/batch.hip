@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "batch.h"
+#include "elabatch.h"
 #include "post.h"
 #include "tb.h"
 #include "tswrtm.h"
@@ -167,7 +168,13 @@ struct Batch {
   };
   std::vector<TbLaunch> tbl;
   DevBuf d_tbshots, d_tbent, d_tbpre;
+  // caller's shot index -> lane (0 acoustic, 1 elastic) and index in the lane
+  int ntotal = 0;
+  std::vector<int> lane_of, local_of;
+  std::vector<ShotHost> ehs;  // elastic shots' geometry (stacking)
+  ElaLane* ela = nullptr;
   ~Batch() {
+    if (ela) ela_lane_destroy(ela);
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto e : graphs) (void)hipGraphExecDestroy(e);
     for (auto gdef : graph_defs) (void)hipGraphDestroy(gdef);
@@ -436,6 +443,11 @@ using namespace tsw;
 
 extern "C" {
 
+static int batch_create_acoustic(Batch* B, int nshot, const tsw_shot_desc* shots, double dx,
+                                 double dz, double dt, int n_sponge, double sponge_alpha,
+                                 int free_surface_top, int k, int slots, int graph_steps,
+                                 int kernel);
+
 int tsw_batch_create(int nshot, const tsw_shot_desc* shots, double dx, double dz, double dt,
                      int n_sponge, double sponge_alpha, int free_surface_top, int k, int slots,
                      int graph_steps, int kernel, tsw_batch_t* out) {
@@ -445,23 +457,64 @@ int tsw_batch_create(int nshot, const tsw_shot_desc* shots, double dx, double dz
   TSW_ARG(k >= 1 && n_sponge >= 0 && dx > 0 && dz > 0 && dt > 0, "bad batch constants");
   TSW_ARG(kernel >= 0 && kernel <= kNumTbCfgs, "kernel %d outside [0, %d]", kernel, kNumTbCfgs);
   *out = nullptr;
+  std::vector<tsw_shot_desc> ac;
+  std::vector<const tsw_shot_desc*> el;
+  std::vector<int> lane_of(nshot), local_of(nshot);
   for (int i = 0; i < nshot; ++i) {
     const tsw_shot_desc& d = shots[i];
+    TSW_ARG(d.kind == TSW_SHOT_ACOUSTIC || d.kind == TSW_SHOT_ELASTIC, "shot %d: bad kind", i);
     TSW_ARG(d.nx >= 5 && d.nz >= 5, "shot %d: grid %d x %d too small", i, d.nx, d.nz);
-    TSW_ARG(d.nt >= 1 && d.vel_fwd && d.vel_bwd && d.src, "shot %d: missing arrays", i);
+    TSW_ARG(d.nt >= 1 && d.src, "shot %d: missing source", i);
     TSW_ARG(d.six >= 0 && d.six < d.nx && d.siz >= 0 && d.siz < d.nz,
             "shot %d: source outside grid", i);
     TSW_ARG(d.riz >= 0 && d.riz < d.nz && d.nrec >= 0, "shot %d: bad receiver row", i);
     TSW_ARG(d.nrec == 0 || (d.rix && d.data), "shot %d: receivers need rix and data", i);
     TSW_ARG((long long)d.nx * d.nz < (1LL << 31), "shot %d: grid too large", i);
+    if (d.kind == TSW_SHOT_ACOUSTIC) {
+      TSW_ARG(d.vel_fwd && d.vel_bwd, "shot %d: missing velocity models", i);
+      lane_of[i] = 0;
+      local_of[i] = (int)ac.size();
+      ac.push_back(d);
+    } else {
+      for (int j = 0; j < 4; ++j)
+        TSW_ARG(d.ela_fwd[j] && d.ela_bwd[j], "shot %d: missing elastic model", i);
+      lane_of[i] = 1;
+      local_of[i] = (int)el.size();
+      el.push_back(&d);
+    }
   }
   Ctx* ctx;
   TSW_TRY(ctx_get(&ctx));
   auto* B = new Batch();
-  auto fail = [&](int rc) {
+  B->ntotal = nshot;
+  B->lane_of = lane_of;
+  B->local_of = local_of;
+  B->kernel = kernel;
+  B->k = k;
+  int rc = kOk;
+  if (!ac.empty())
+    rc = batch_create_acoustic(B, (int)ac.size(), ac.data(), dx, dz, dt, n_sponge, sponge_alpha,
+                               free_surface_top, k, slots, graph_steps, kernel);
+  if (rc == kOk && !el.empty()) {
+    rc = ela_lane_create(el, dx, dz, dt, n_sponge, sponge_alpha, free_surface_top, k, slots,
+                         graph_steps, &B->ela);
+    for (const auto* d : el)
+      B->ehs.push_back(ShotHost{d->nx, d->nz, d->nt, d->six, d->siz, d->riz, d->nrec, d->x0,
+                                d->z0, 0});
+  }
+  if (rc != kOk) {
     delete B;
     return rc;
-  };
+  }
+  *out = reinterpret_cast<tsw_batch_t>(B);
+  return kOk;
+}
+
+static int batch_create_acoustic(Batch* B, int nshot, const tsw_shot_desc* shots, double dx,
+                                 double dz, double dt, int n_sponge, double sponge_alpha,
+                                 int free_surface_top, int k, int slots, int graph_steps,
+                                 int kernel) {
+  auto fail = [&](int rc) { return rc; };  // the caller deletes the batch
   B->nshot = nshot;
   B->k = k;
   B->dt = dt;
@@ -571,14 +624,10 @@ int tsw_batch_create(int nshot, const tsw_shot_desc* shots, double dx, double dz
   if ((rc = build_epochs(B))) return fail(rc);
   if (B->kernel > 0 && (rc = build_tb(B))) return fail(rc);
   if (B->graph_steps > 0 && (rc = capture_graphs(B))) return fail(rc);
-  *out = reinterpret_cast<tsw_batch_t>(B);
   return kOk;
 }
 
-int tsw_batch_run(tsw_batch_t h, int mode) {
-  Batch* B = reinterpret_cast<Batch*>(h);
-  TSW_ARG(B, "null batch");
-  TSW_TRY(fail_injected());
+static int run_acoustic(Batch* B, int mode) {
   hipStream_t st = B->stream;
   bool per_step = true;
   if (mode == TSW_RUN_BATCHED) {
@@ -588,7 +637,7 @@ int tsw_batch_run(tsw_batch_t h, int mode) {
       issue(B, 0, launches_total(B), st);
     }
     per_step = B->kernel == 0;
-  } else if (mode == TSW_RUN_PLAIN) {
+  } else {
     // one shot at a time, one launch per step covering only that shot
     DevBuf one;
     int lst[3];
@@ -606,22 +655,38 @@ int tsw_batch_run(tsw_batch_t h, int mode) {
       for (int m = -1; m < 2 * B->hs[i].nt; ++m) launch_step(B, e, B->sd[i].start + m, st);
       TSW_HIP(hipStreamSynchronize(st));
     }
-  } else {
-    TSW_ARG(false, "unknown run mode %d", mode);
   }
   // the per-step kernel leaves illumination unscaled; the blocked kernel
   // scales it in the last forward block
   if (per_step) TSW_TRY(finish_shots(B, st));
   TSW_HIP(hipGetLastError());
-  TSW_HIP(hipStreamSynchronize(st));
   return kOk;
 }
 
-int tsw_batch_time_kernels(tsw_batch_t h, double* kernel_ms, int* launches) {
+int tsw_batch_run(tsw_batch_t h, int mode) {
+  Batch* B = reinterpret_cast<Batch*>(h);
+  TSW_ARG(B, "null batch");
+  TSW_ARG(mode == TSW_RUN_BATCHED || mode == TSW_RUN_PLAIN, "unknown run mode %d", mode);
+  TSW_TRY(fail_injected());
+  // the elastic lane is enqueued first on its own stream so both lanes overlap
+  if (B->ela) TSW_TRY(ela_lane_launch(B->ela, mode));
+  if (B->nshot > 0) TSW_TRY(run_acoustic(B, mode));
+  if (B->nshot > 0) TSW_HIP(hipStreamSynchronize(B->stream));
+  if (B->ela) TSW_TRY(ela_lane_sync(B->ela));
+  return kOk;
+}
+
+int tsw_batch_time_kernels(tsw_batch_t h, int lane, double* kernel_ms, int* launches) {
   Batch* B = reinterpret_cast<Batch*>(h);
   TSW_ARG(B && kernel_ms && launches, "null argument");
+  TSW_ARG(lane == 0 || lane == 1, "lane must be 0 (acoustic) or 1 (elastic)");
+  if (lane == 1) {
+    TSW_ARG(B->ela, "batch has no elastic shots");
+    return ela_lane_time(B->ela, kernel_ms, launches);
+  }
+  TSW_ARG(B->nshot > 0, "batch has no acoustic shots");
   hipStream_t st = B->stream;
-  // event pair around every step launch, recorded in chunks to bound the pool
+  // event pair around every launch, recorded in chunks to bound the pool
   const int chunk = 512;
   std::vector<hipEvent_t> ev(2 * chunk);
   for (auto& e : ev) TSW_HIP(hipEventCreate(&e));
@@ -664,34 +729,50 @@ int tsw_batch_time_kernels(tsw_batch_t h, double* kernel_ms, int* launches) {
   return kOk;
 }
 
+static const ShotHost& shot_host(const Batch* B, int i) {
+  return B->lane_of[i] == 0 ? B->hs[B->local_of[i]] : B->ehs[B->local_of[i]];
+}
+static const float* shot_image(Batch* B, int i) {
+  const int l = B->local_of[i];
+  return B->lane_of[i] == 0 ? B->sd[l].image : ela_lane_image(B->ela, l);
+}
+static const float* shot_illum(Batch* B, int i) {
+  const int l = B->local_of[i];
+  return B->lane_of[i] == 0 ? B->sd[l].illum : ela_lane_illum(B->ela, l);
+}
+
 int tsw_batch_shot_result(tsw_batch_t h, int shot, float* image_out, float* illum_out) {
   Batch* B = reinterpret_cast<Batch*>(h);
-  TSW_ARG(B && shot >= 0 && shot < B->nshot, "bad shot index");
-  const size_t nb = (size_t)B->hs[shot].cells() * sizeof(float);
-  if (image_out) TSW_HIP(hipMemcpy(image_out, B->sd[shot].image, nb, hipMemcpyDeviceToHost));
-  if (illum_out) TSW_HIP(hipMemcpy(illum_out, B->sd[shot].illum, nb, hipMemcpyDeviceToHost));
+  TSW_ARG(B && shot >= 0 && shot < B->ntotal, "bad shot index");
+  const size_t nb = (size_t)shot_host(B, shot).cells() * sizeof(float);
+  if (image_out) TSW_HIP(hipMemcpy(image_out, shot_image(B, shot), nb, hipMemcpyDeviceToHost));
+  if (illum_out) TSW_HIP(hipMemcpy(illum_out, shot_illum(B, shot), nb, hipMemcpyDeviceToHost));
   return kOk;
 }
 
 int tsw_batch_stack(tsw_batch_t h, int gnx, int gnz, float* image_out, float* illum_out) {
   Batch* B = reinterpret_cast<Batch*>(h);
   TSW_ARG(B && gnx >= 1 && gnz >= 1 && image_out && illum_out, "bad argument");
-  for (const auto& s : B->hs)
+  for (int i = 0; i < B->ntotal; ++i) {
+    const ShotHost& s = shot_host(B, i);
     TSW_ARG(s.x0 >= 0 && s.z0 >= 0 && s.x0 + s.nx <= gnx && s.z0 + s.nz <= gnz,
             "shot window outside the %d x %d stack", gnx, gnz);
+  }
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  hipStream_t st = ctx->stream;
   const long long n = (long long)gnx * gnz;
   DevBuf img, ill;
   TSW_TRY(img.ensure(n * sizeof(float)));
   TSW_TRY(ill.ensure(n * sizeof(float)));
-  hipStream_t st = B->stream;
   launch_fill(img.f(), n, 0.0f, st);
   launch_fill(ill.f(), n, 0.0f, st);
-  for (int i = 0; i < B->nshot; ++i) {  // shot order, as the reference's stack loop
-    const auto& s = B->hs[i];
-    launch_stack_window(img.f() + (long long)s.z0 * gnx, gnx, B->sd[i].image, s.nx, s.nz, s.x0, 0,
-                        st);
-    launch_stack_window(ill.f() + (long long)s.z0 * gnx, gnx, B->sd[i].illum, s.nx, s.nz, s.x0, 0,
-                        st);
+  for (int i = 0; i < B->ntotal; ++i) {  // shot order, as the reference's stack loop
+    const ShotHost& s = shot_host(B, i);
+    launch_stack_window(img.f() + (long long)s.z0 * gnx, gnx, shot_image(B, i), s.nx, s.nz, s.x0,
+                        0, st);
+    launch_stack_window(ill.f() + (long long)s.z0 * gnx, gnx, shot_illum(B, i), s.nx, s.nz, s.x0,
+                        0, st);
   }
   TSW_HIP(hipGetLastError());
   TSW_HIP(hipMemcpyAsync(image_out, img.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -701,13 +782,17 @@ int tsw_batch_stack(tsw_batch_t h, int gnx, int gnz, float* image_out, float* il
 }
 
 int tsw_batch_info(tsw_batch_t h, long long* cell_updates, long long* device_bytes,
-                   int* total_steps, int* n_graphs) {
+                   int* total_steps, int* n_graphs, long long* elastic_cell_updates) {
   Batch* B = reinterpret_cast<Batch*>(h);
   TSW_ARG(B, "null batch");
-  if (cell_updates) *cell_updates = B->cell_updates;
-  if (device_bytes) *device_bytes = (long long)B->device_bytes;
-  if (total_steps) *total_steps = B->total_steps;
-  if (n_graphs) *n_graphs = (int)B->graphs.size();
+  const long long ecu = B->ela ? ela_lane_cell_updates(B->ela) : 0;
+  if (cell_updates) *cell_updates = B->cell_updates + ecu;
+  if (device_bytes)
+    *device_bytes = (long long)B->device_bytes + (B->ela ? ela_lane_device_bytes(B->ela) : 0);
+  if (total_steps)
+    *total_steps = std::max(B->total_steps, B->ela ? ela_lane_steps(B->ela) : 0);
+  if (n_graphs) *n_graphs = (int)B->graphs.size() + (B->ela ? ela_lane_graphs(B->ela) : 0);
+  if (elastic_cell_updates) *elastic_cell_updates = ecu;
   return kOk;
 }
 

@@ -68,6 +68,28 @@ class Shot:
 
 
 @dataclass
+class ElasticShot:
+    """One elastic virtual shot: models are (lambda, mu, buoyancy_x, buoyancy_z)."""
+    nx: int
+    nz: int
+    nt: int
+    model_fwd: tuple
+    model_bwd: tuple
+    src: np.ndarray
+    six: int
+    siz: int
+    data: np.ndarray            # (nt, nrec) vertical strain rate
+    rix: np.ndarray
+    riz: int
+    x0: int = 0
+    kind: str = "elastic"
+
+    @property
+    def cells(self):
+        return self.nx * self.nz
+
+
+@dataclass
 class Config0:
     """Plain acoustic RTM, one virtual shot: 200 x 400 at 5 m, dt 0.5 ms, nt 4000."""
     nz: int = 200

@@ -42,7 +42,13 @@ typedef struct tsw_shot_desc {
   const int* rix;          /* (nrec) receiver columns on row riz */
   int nrec, riz;
   int x0, z0;              /* window offset in the stacking grid */
+  int kind;                /* TSW_SHOT_ACOUSTIC or TSW_SHOT_ELASTIC */
+  const float* ela_fwd[4]; /* elastic shots: forward lambda, mu, buoyancy_x, buoyancy_z */
+  const float* ela_bwd[4]; /* elastic shots: back-propagation model */
 } tsw_shot_desc;
+
+#define TSW_SHOT_ACOUSTIC 0
+#define TSW_SHOT_ELASTIC 1
 
 #define TSW_RUN_BATCHED 0  /* all active shots in one launch per step (graphs) */
 #define TSW_RUN_PLAIN 1    /* one shot at a time, one launch per step */
@@ -159,9 +165,11 @@ int tsw_migrate_shot_elastic(tsw_grid_t grid, const float* const* model_fwd,
                              const float* data, int nt, const int* rix, int nrec, int riz, int k,
                              float* image_out, float* illum_out);
 
-/* Batched acoustic RTM (no reference counterpart; it batches what
+/* Batched RTM (no reference counterpart; it batches what
  * TSWRTMEngine.migrate_multiple_shots, rtm_engine.py:125-143, runs one shot at
- * a time).  Uploads every shot's models, source and data, sizes the HBM state
+ * a time).  Acoustic shots form the acoustic lane and elastic shots
+ * (kind TSW_SHOT_ELASTIC, vel_* unused) the elastic lane; the two lanes run on
+ * their own streams and overlap.  Uploads every shot's models, source and data, sizes the HBM state
  * and captures the time loop into hipGraphs of `graph_steps` time steps
  * (0: no graphs).  Shots are packed into `slots` concurrent lanes (<= 0: all
  * shots at once).  kernel 0: one launch per time step over the tiles of every
@@ -177,17 +185,19 @@ int tsw_kernel_configs(void);
 int tsw_kernel_config(int id, int* T, int* W, int* H, int* threads);
 /* Migrates every shot (mode TSW_RUN_BATCHED or TSW_RUN_PLAIN); synchronous. */
 int tsw_batch_run(tsw_batch_t batch, int mode);
-/* Runs the batch once with a HIP event pair around every step launch (no
- * graphs); returns the summed kernel time and the number of launches. */
-int tsw_batch_time_kernels(tsw_batch_t batch, double* kernel_ms, int* launches);
+/* Runs one lane of the batch (0 acoustic, 1 elastic) once with a HIP event
+ * pair around every launch (no graphs); returns the summed kernel time and the
+ * number of launches. */
+int tsw_batch_time_kernels(tsw_batch_t batch, int lane, double* kernel_ms, int* launches);
 /* Per-shot image and illumination (nz, nx) of the last run; either may be NULL. */
 int tsw_batch_shot_result(tsw_batch_t batch, int shot, float* image_out, float* illum_out);
 /* FP32 stack, in shot order, of every shot's image and illumination into a
  * (gnz, gnx) grid at the shots' window offsets. */
 int tsw_batch_stack(tsw_batch_t batch, int gnx, int gnz, float* image_out, float* illum_out);
-/* Cell-updates per run, device bytes held, global steps, captured graphs. */
+/* Cell-updates per run, device bytes held, global steps, captured graphs
+ * (both lanes), and the elastic lane's share of the cell-updates. */
 int tsw_batch_info(tsw_batch_t batch, long long* cell_updates, long long* device_bytes,
-                   int* total_steps, int* n_graphs);
+                   int* total_steps, int* n_graphs, long long* elastic_cell_updates);
 int tsw_batch_destroy(tsw_batch_t batch);
 
 #ifdef __cplusplus

@@ -120,3 +120,52 @@ def test_blocked_kernels_equal_single_shots(mixed):
                 assert nl > 0 and ms > 0
                 _check(b, shots, singles[fs], f"kernel {kid} timed run")
                 b.close()
+
+
+def test_mixed_acoustic_elastic_batch(mixed):
+    """Elastic shots form their own lane; acoustic and elastic shots of one
+    batch each equal their single-shot migration bit for bit."""
+    from das_amd.batch import ShotBatch
+    from das_amd.rtm_engine import TSWRTMEngine
+    from das_amd.wave_solver import WaveSolver2D
+    from das_amd.workloads import ElasticShot, elastic_from_vp, layered_faulted_vp, smooth
+
+    shots, gnx, gnz, singles = mixed
+    rng = np.random.default_rng(5)
+    eshots, esingle = [], []
+    for nz, nx, nt, six, riz in ((40, 70, 50, 30, 2), (56, 100, 41, 60, 1)):
+        vp = layered_faulted_vp(nz, nx, DX, seed=int(rng.integers(50)))
+        mt = elastic_from_vp(vp)
+        mm = elastic_from_vp(smooth(vp, 40.0, DX))
+        src = (rng.standard_normal(nt) * 1e3).astype(np.float32)
+        rix = np.arange(1, nx - 1, 3).astype(np.int32)
+        solver = WaveSolver2D(nx, nz, DX, DX, 4e-4, n_sponge=NSP, solver_type="elastic")
+        data, _ = solver.forward_propagate_elastic(mt, src, six, 2, rix, riz)
+        eshots.append(ElasticShot(nx=nx, nz=nz, nt=nt, model_fwd=mm, model_bwd=mt, src=src,
+                                  six=six, siz=2, data=data, rix=rix, riz=riz, x0=3))
+        esingle.append(TSWRTMEngine(solver).migrate_shot_elastic(mm, mt, src, six, 2, data, rix,
+                                                                  riz, K))
+    # the elastic shots use dt 4e-4; pair them with acoustic shots of the same dt
+    from das_amd.workloads import Shot
+    ashot = shots[1]
+    a = Shot(**{**ashot.__dict__})
+    solver = WaveSolver2D(a.nx, a.nz, DX, DX, 4e-4, n_sponge=NSP)
+    from das_amd.velocity_flood import FloodModels
+    asingle = TSWRTMEngine(solver).migrate_shot(FloodModels(a.vel_fwd, a.vel_fwd, a.vel_bwd, 0, 0),
+                                                a.src, a.six, a.siz, a.data, a.rix, a.riz, K)
+    order = [eshots[0], a, eshots[1]]
+    expect = [esingle[0], asingle, esingle[1]]
+    for kernel, slots, gs in ((0, 0, 100), (11, 1, 0), (7, 2, 30)):
+        b = ShotBatch(order, DX, DX, 4e-4, n_sponge=NSP, k=K, slots=slots, kernel=kernel,
+                      graph_steps=gs)
+        for mode in ("batched", "plain"):
+            b.run(mode)
+            for i, (img, ill) in enumerate(expect):
+                bi, bl = b.shot_result(i)
+                np.testing.assert_array_equal(bi, img, err_msg=f"k{kernel} {mode} shot {i}")
+                np.testing.assert_array_equal(bl, ill, err_msg=f"k{kernel} {mode} shot {i}")
+        ms, nl = b.time_kernels("elastic")
+        assert nl > 0 and ms > 0
+        info = b.info()
+        assert info["elastic_cell_updates"] == sum(2 * s.nt * s.nx * s.nz for s in eshots)
+        b.close()
