@@ -55,6 +55,14 @@ SIGNATURES = {
     "tsw_batch_info": (I, [P, ctypes.POINTER(LL), ctypes.POINTER(LL), ctypes.POINTER(I),
                            ctypes.POINTER(I), ctypes.POINTER(LL)]),
     "tsw_batch_destroy": (I, [P]),
+    "tsw_server_create": (I, [D, D, D, I, D, I, I, LL, I, I, I, ctypes.POINTER(P)]),
+    "tsw_server_submit": (I, [P, P, ctypes.POINTER(LL)]),
+    "tsw_server_poll": (I, [P, LL, ctypes.POINTER(I)] + [ctypes.POINTER(D)] * 3),
+    "tsw_server_wait": (I, [P, LL, D]),
+    "tsw_server_result": (I, [P, LL, P, P]),
+    "tsw_server_cancel": (I, [P, LL]),
+    "tsw_server_stats": (I, [P] + [ctypes.POINTER(LL)] * 7 + [ctypes.POINTER(D)]),
+    "tsw_server_destroy": (I, [P]),
 }
 
 RUN_BATCHED = 0

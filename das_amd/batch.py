@@ -15,6 +15,39 @@ import numpy as np
 from . import _native as N
 
 
+def fill_desc(d, s, keep):
+    """Fills tsw_shot_desc `d` from a shot object; arrays it points to are
+    appended to `keep` (they must outlive the C call)."""
+    src = N.f32(np.asarray(s.src).reshape(-1))
+    data = N.f32(s.data)
+    rix = N.i32(s.rix)
+    if len(src) != s.nt or data.shape != (s.nt, len(rix)):
+        raise ValueError("source / data length must equal the shot's nt")
+    keep += [src, data, rix]
+    if getattr(s, "kind", "acoustic") == "elastic":
+        mf = [N.f32(a) for a in s.model_fwd]
+        mb = [N.f32(a) for a in s.model_bwd]
+        if len(mf) != 4 or len(mb) != 4 or any(a.shape != (s.nz, s.nx) for a in mf + mb):
+            raise ValueError("elastic model shape does not match the shot grid")
+        keep += mf + mb
+        d.kind = N.SHOT_ELASTIC
+        for j in range(4):
+            d.ela_fwd[j] = N.ptr(mf[j])
+            d.ela_bwd[j] = N.ptr(mb[j])
+    else:
+        vf, vb = N.f32(s.vel_fwd), N.f32(s.vel_bwd)
+        if vf.shape != (s.nz, s.nx) or vb.shape != (s.nz, s.nx):
+            raise ValueError("model shape does not match the shot grid")
+        keep += [vf, vb]
+        d.kind = N.SHOT_ACOUSTIC
+        d.vel_fwd, d.vel_bwd = N.ptr(vf), N.ptr(vb)
+    d.nx, d.nz, d.nt = s.nx, s.nz, s.nt
+    d.src = N.ptr(src)
+    d.six, d.siz = int(s.six), int(s.siz)
+    d.data, d.rix, d.nrec, d.riz = N.ptr(data), N.ptr(rix), len(rix), int(s.riz)
+    d.x0, d.z0 = int(getattr(s, "x0", 0)), int(getattr(s, "z0", 0))
+
+
 def kernel_configs():
     """{id: (T, W, H, threads)} of the temporally blocked kernel."""
     out = {}
@@ -37,34 +70,7 @@ class ShotBatch:
         keep = []
         descs = (N.ShotDesc * len(self.shots))()
         for d, s in zip(descs, self.shots):
-            src = N.f32(np.asarray(s.src).reshape(-1))
-            data = N.f32(s.data)
-            rix = N.i32(s.rix)
-            if len(src) != s.nt or data.shape != (s.nt, len(rix)):
-                raise ValueError("source / data length must equal the shot's nt")
-            keep += [src, data, rix]
-            if getattr(s, "kind", "acoustic") == "elastic":
-                mf = [N.f32(a) for a in s.model_fwd]
-                mb = [N.f32(a) for a in s.model_bwd]
-                if any(a.shape != (s.nz, s.nx) for a in mf + mb) or len(mf) != 4 or len(mb) != 4:
-                    raise ValueError("elastic model shape does not match the shot grid")
-                keep += mf + mb
-                d.kind = N.SHOT_ELASTIC
-                for j in range(4):
-                    d.ela_fwd[j] = N.ptr(mf[j])
-                    d.ela_bwd[j] = N.ptr(mb[j])
-            else:
-                vf, vb = N.f32(s.vel_fwd), N.f32(s.vel_bwd)
-                if vf.shape != (s.nz, s.nx) or vb.shape != (s.nz, s.nx):
-                    raise ValueError("model shape does not match the shot grid")
-                keep += [vf, vb]
-                d.kind = N.SHOT_ACOUSTIC
-                d.vel_fwd, d.vel_bwd = N.ptr(vf), N.ptr(vb)
-            d.nx, d.nz, d.nt = s.nx, s.nz, s.nt
-            d.src = N.ptr(src)
-            d.six, d.siz = int(s.six), int(s.siz)
-            d.data, d.rix, d.nrec, d.riz = N.ptr(data), N.ptr(rix), len(rix), int(s.riz)
-            d.x0, d.z0 = int(getattr(s, "x0", 0)), int(getattr(s, "z0", 0))
+            fill_desc(d, s, keep)
         h = N.P()
         alpha32 = float(np.float32(sponge_alpha))
         N.check(N.lib().tsw_batch_create(len(self.shots), descs, self.dx, self.dz, self.dt,

@@ -229,6 +229,20 @@ static const TbCfg kTbCfgs[] = {
 };
 constexpr int kNumTbCfgs = (int)(sizeof(kTbCfgs) / sizeof(kTbCfgs[0]));
 
+int tb_config(int id, int* T, int* W, int* H) {
+  if (id < 1 || id > kNumTbCfgs) return kErrArg;
+  const TbCfg& c = kTbCfgs[id - 1];
+  *T = c.T;
+  *W = c.W;
+  *H = c.H;
+  return kOk;
+}
+
+void tb_launch(int id, const BatchConst& bc, const TbShot* shots, const TbEntry* ents,
+               const int* pre, int nent, int ntiles, hipStream_t st) {
+  kTbCfgs[id - 1].fn(bc, shots, ents, pre, nent, ntiles, st);
+}
+
 static void launch_tb_rec(Batch* B, const Batch::TbLaunch& L, hipStream_t st) {
   const TbCfg& c = kTbCfgs[B->kernel - 1];
   c.fn(B->bc, (const TbShot*)B->d_tbshots.p, (const TbEntry*)B->d_tbent.p + L.ent_off,

@@ -7,6 +7,36 @@
 
 namespace tsw {
 
+struct EShot {
+  int nx, nz, nt, k, start, src_cell, riz, nrec;
+  float* f[5];          // forward vx, vz, sxx, szz, sxz
+  float* fb[5];         // back-propagation vx, vz, sxx, szz, sxz
+  const float* mf[4];   // forward model: lambda, mu, bx, bz
+  const float* mb[4];   // back-propagation model
+  const float* src;
+  const float* data;
+  const int* roff;
+  const int* ridx;
+  float* snaps;
+  float* illum;
+  float* image;
+  int tiles_x, tiles;
+};
+
+struct EConst {
+  int fs, n_sponge;
+  float rdx, rdz, dt, w;
+  const float* table;
+};
+
+// One launch of the batched elastic step (pass 0 velocities, 1 stresses) over
+// the shots in `list` (ids, then tile prefix) at global step g; shot time is
+// m = g - start (m < 0: initialisation).
+void ela_batch_launch(const EConst& c, const EShot* shots, const int* list, int nact,
+                      int ntiles, int g, int pass, hipStream_t st);
+int ela_tiles(int nx, int nz, int* tiles_x);
+void ela_scale_launch(float* p, long long n, float w, hipStream_t st);
+
 struct ElaLane;
 
 int ela_lane_create(const std::vector<const tsw_shot_desc*>& shots, double dx, double dz,

@@ -18,28 +18,6 @@ namespace tsw {
 constexpr int kEBX = 64;
 constexpr int kEBZ = 4;
 
-struct EShot {
-  int nx, nz, nt, k, start, src_cell, riz, nrec;
-  float* f[5];          // forward vx, vz, sxx, szz, sxz
-  float* fb[5];         // back-propagation vx, vz, sxx, szz, sxz
-  const float* mf[4];   // forward model: lambda, mu, bx, bz
-  const float* mb[4];   // back-propagation model
-  const float* src;
-  const float* data;
-  const int* roff;
-  const int* ridx;
-  float* snaps;
-  float* illum;
-  float* image;
-  int tiles_x, tiles;
-};
-
-struct EConst {
-  int fs, n_sponge;
-  float rdx, rdz, dt, w;
-  const float* table;
-};
-
 __device__ __forceinline__ int xcd_tile_e(int b, int n) {
   const int per = n >> 3, rem = n & 7;
   const int x = b & 7, l = b >> 3;
@@ -135,6 +113,17 @@ __global__ __launch_bounds__(kEBX* kEBZ) void k_ela_batch(EConst c, const EShot*
   s.bz = model[3];
   if (pass == 0) ela_vel_cell(s, ix, iz);
   else ela_str_cell(s, ix, iz);
+}
+
+void ela_batch_launch(const EConst& c, const EShot* shots, const int* list, int nact,
+                      int ntiles, int g, int pass, hipStream_t st) {
+  hipLaunchKernelGGL(k_ela_batch, dim3(ntiles), dim3(kEBX, kEBZ), 0, st, c, shots, list, nact,
+                     ntiles, g, pass);
+}
+
+int ela_tiles(int nx, int nz, int* tiles_x) {
+  *tiles_x = (nx + kEBX - 1) / kEBX;
+  return *tiles_x * ((nz + kEBZ - 1) / kEBZ);
 }
 
 __global__ void k_ela_scale(float* p, long long n, float w) {
@@ -244,6 +233,11 @@ static int e_build(ElaLane* L) {
                     hipMemcpyHostToDevice));
   for (size_t j = 0; j < L->epochs.size(); ++j) L->epochs[j].d_list = L->d_lists.i() + offs[j];
   return kOk;
+}
+
+void ela_scale_launch(float* p, long long n, float w, hipStream_t st) {
+  hipLaunchKernelGGL(k_ela_scale, dim3((unsigned)std::min<long long>((n + 255) / 256, 4096)),
+                     dim3(256), 0, st, p, n, w);
 }
 
 static int e_finish(ElaLane* L, hipStream_t st) {

@@ -1,0 +1,627 @@
+// Serving loop for virtual-shot migration requests (continuous batching).
+//
+// Requests (acoustic or elastic shots of any grid size and length) are
+// submitted at any time and queued in arrival order.  A worker thread owns the
+// device: at every tick it admits queued requests while a slot is free and the
+// request's HBM footprint (models, wavefields, snapshot arena, image) fits the
+// budget, then advances every running request together: `tick_steps` time
+// steps, as temporally blocked acoustic launches on one stream and per-step
+// elastic launches on a second stream, so the two kinds overlap on the GPU.
+// A request that finishes releases its wavefields and snapshots at once and
+// keeps its image and illumination until the caller fetches them.
+//
+// Every request runs through the same device functions as the single-shot
+// paths, with no cross-request arithmetic, so its result is bit-identical
+// alone and in any mix.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "elabatch.h"
+#include "post.h"
+#include "tb.h"
+#include "tswrtm.h"
+
+namespace tsw {
+
+enum ReqState { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3, kCancelled = 4 };
+
+struct Req {
+  long long id = 0;
+  int kind = TSW_SHOT_ACOUSTIC;
+  int nx = 0, nz = 0, nt = 0, six = 0, siz = 0, riz = 0, nrec = 0;
+  std::vector<float> model[8];  // acoustic: [0] fwd vel, [1] bwd vel; elastic: 4 fwd + 4 bwd
+  std::vector<float> src, data;
+  std::vector<int> rix;
+  int state = kQueued;
+  bool cancel = false;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  long long need = 0;     // device bytes while running
+  long long keep = 0;     // device bytes kept after completion (image + illumination)
+  int slot = -1;
+  int cursor = 0;         // acoustic: next block; elastic: unused (global step based)
+  int e_start = 0;        // elastic: global elastic step of the first forward step
+  std::vector<DevBuf> heavy;   // released at completion
+  DevBuf image, illum;         // released when fetched
+  std::string err;
+  long long cells() const { return (long long)nx * nz; }
+};
+
+struct Server {
+  // constants
+  double dt = 0;
+  int k = 2, kernel = 0, T = 1, W = 0, H = 0, tick_steps = 100, max_active = 1;
+  long long budget = 0;
+  BatchConst bc{};
+  EConst ec{};
+  hipStream_t sa = nullptr, se = nullptr;
+  DevBuf d_table, d_tbshots, d_eshots, d_tbent, d_tbpre, d_elist;
+  std::vector<TbShot> tbs;
+  std::vector<EShot> es;
+  // state
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Req*> queue;
+  std::map<long long, std::unique_ptr<Req>> reqs;
+  std::vector<Req*> slots;
+  long long mem_used = 0, next_id = 1;
+  int g_e = 0;
+  bool stop = false;
+  std::thread worker;
+  std::chrono::steady_clock::time_point t0;
+  // statistics
+  long long completed = 0, failed = 0, cancelled = 0, cell_updates = 0;
+  double busy_s = 0;
+  std::string fatal;
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+static long long req_need(const Req& r, int k) {
+  const long long n = r.cells();
+  const long long nsnap = (r.nt - 1) / k + 1;
+  const long long io = 4LL * (r.nt + (long long)r.nt * std::max(r.nrec, 1)) + 4LL * (r.nx + 1 + r.nrec);
+  // models, wavefields, snapshot arena, data; image + illumination (2 n) included
+  if (r.kind == TSW_SHOT_ACOUSTIC) return 4 * n * (2 + 6 + 2) + 4 * nsnap * n + io;
+  return 4 * n * (8 + 10 + 2) + 4 * nsnap * n + io;
+}
+
+static int alloc_up(DevBuf& d, const void* h, size_t bytes, hipStream_t st) {
+  TSW_TRY(d.ensure(bytes ? bytes : 4));
+  if (h && bytes) TSW_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, st));
+  return kOk;
+}
+
+// Allocates and uploads a request's device state into slot `s`.
+static int admit(Server* S, Req* r, int s) {
+  TSW_TRY(fail_injected());
+  hipStream_t st = r->kind == TSW_SHOT_ACOUSTIC ? S->sa : S->se;
+  const long long n = r->cells();
+  const size_t nb = (size_t)n * sizeof(float);
+  const long long nsnap = (r->nt - 1) / S->k + 1;
+  RecvMap m;
+  TSW_TRY(m.build(r->rix.data(), r->nrec, r->nx));
+  r->heavy.clear();
+  r->heavy.resize(r->kind == TSW_SHOT_ACOUSTIC ? 13 : 23);
+  auto& H = r->heavy;
+  TSW_TRY(alloc_up(H[0], r->src.data(), r->src.size() * sizeof(float), st));
+  TSW_TRY(alloc_up(H[1], r->data.data(), r->data.size() * sizeof(float), st));
+  TSW_TRY(alloc_up(H[2], m.off.data(), m.off.size() * sizeof(int), st));
+  TSW_TRY(alloc_up(H[3], m.idx.data(), m.idx.size() * sizeof(int), st));
+  TSW_TRY(r->image.ensure(nb));
+  TSW_TRY(r->illum.ensure(nb));
+  if (r->kind == TSW_SHOT_ACOUSTIC) {
+    // [4],[5] coefficients; [6..11] two sets of three levels; [12] snapshots
+    DevBuf vel;
+    for (int j = 0; j < 2; ++j) {
+      TSW_TRY(alloc_up(vel, r->model[j].data(), nb, st));
+      TSW_TRY(H[4 + j].ensure(nb));
+      const float dtf = (float)S->dt;
+      launch_coef(vel.f(), H[4 + j].f(), n, dtf * dtf, st);
+      TSW_HIP(hipStreamSynchronize(st));  // `vel` is reused / freed
+    }
+    for (int j = 6; j < 12; ++j) TSW_TRY(H[j].ensure(nb));
+    TSW_TRY(H[12].ensure((size_t)nsnap * nb));
+    TbShot P;
+    std::memset(&P, 0, sizeof P);
+    P.nx = r->nx;
+    P.nz = r->nz;
+    P.nt = r->nt;
+    P.k = S->k;
+    P.src_cell = r->siz * r->nx + r->six;
+    P.riz = r->riz;
+    P.nrec = r->nrec;
+    for (int j = 0; j < 3; ++j) {
+      P.set[0][j] = H[6 + j].f();
+      P.set[1][j] = H[9 + j].f();
+    }
+    P.coef_f = H[4].f();
+    P.coef_b = H[5].f();
+    P.src = H[0].f();
+    P.data = H[1].f();
+    P.roff = H[2].i();
+    P.ridx = H[3].i();
+    P.snaps = H[12].f();
+    P.illum = r->illum.f();
+    P.image = r->image.f();
+    S->tbs[s] = P;
+    TSW_HIP(hipMemcpyAsync((TbShot*)S->d_tbshots.p + s, &S->tbs[s], sizeof(TbShot),
+                           hipMemcpyHostToDevice, st));
+    r->cursor = 0;
+  } else {
+    // [4..11] models (4 fwd, 4 bwd); [12..21] fields; [22] snapshots
+    for (int j = 0; j < 8; ++j) TSW_TRY(alloc_up(H[4 + j], r->model[j].data(), nb, st));
+    for (int j = 12; j < 22; ++j) TSW_TRY(H[j].ensure(nb));
+    TSW_TRY(H[22].ensure((size_t)nsnap * nb));
+    EShot E;
+    std::memset(&E, 0, sizeof E);
+    E.nx = r->nx;
+    E.nz = r->nz;
+    E.nt = r->nt;
+    E.k = S->k;
+    E.start = S->g_e + 1;  // initialisation at the tick's first elastic step
+    E.src_cell = r->siz * r->nx + r->six;
+    E.riz = r->riz;
+    E.nrec = r->nrec;
+    for (int j = 0; j < 5; ++j) {
+      E.f[j] = H[12 + j].f();
+      E.fb[j] = H[17 + j].f();
+    }
+    for (int j = 0; j < 4; ++j) {
+      E.mf[j] = H[4 + j].f();
+      E.mb[j] = H[8 + j].f();
+    }
+    E.src = H[0].f();
+    E.data = H[1].f();
+    E.roff = H[2].i();
+    E.ridx = H[3].i();
+    E.snaps = H[22].f();
+    E.illum = r->illum.f();
+    E.image = r->image.f();
+    E.tiles = ela_tiles(r->nx, r->nz, &E.tiles_x);
+    S->es[s] = E;
+    TSW_HIP(hipMemcpyAsync((EShot*)S->d_eshots.p + s, &S->es[s], sizeof(EShot),
+                           hipMemcpyHostToDevice, st));
+    r->e_start = E.start;
+  }
+  TSW_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+static void release(Server* S, Req* r) {
+  r->heavy.clear();
+  if (r->slot >= 0) S->slots[r->slot] = nullptr;
+  r->slot = -1;
+}
+
+// One tick: tick_steps time steps of every running request.  Runs without
+// the server lock (only the worker touches slots and cursors); requests that
+// finished are returned in `done`, work counters in `cu` / `busy`.
+static int tick(Server* S, std::vector<Req*>& done, long long& cu, double& busy) {
+  cu = 0;
+  busy = 0;
+  std::vector<Req*> act_a, act_e;
+  for (Req* r : S->slots)
+    if (r) (r->kind == TSW_SHOT_ACOUSTIC ? act_a : act_e).push_back(r);
+  // acoustic: blocked launches, each running request advances one block each
+  const int nlaunch = std::max(1, S->tick_steps / S->T);
+  std::vector<TbEntry> ents;
+  std::vector<int> pres;
+  std::vector<std::pair<int, int>> lr;  // (entry offset, prefix offset) per launch
+  std::vector<int> lnent, ltiles;
+  for (int L = 0; L < nlaunch && !act_a.empty(); ++L) {
+    const int eo = (int)ents.size(), po = (int)pres.size();
+    pres.push_back(0);
+    for (Req* r : act_a) {
+      const int nb = (r->nt + S->T - 1) / S->T;
+      if (r->cursor >= 2 * nb) continue;
+      const int b = r->cursor < nb ? r->cursor : r->cursor - nb;
+      const int ph = r->cursor < nb ? 0 : 1;
+      TbEntry e{};
+      e.shot = r->slot;
+      e.phase = ph;
+      e.it0 = ph == 0 ? b * S->T : r->nt - 1 - b * S->T;
+      e.nsub = ph == 0 ? std::min(S->T, r->nt - e.it0) : std::min(S->T, e.it0 + 1);
+      e.flags = (b == 0 ? kTbFirst : 0) | (b == nb - 1 ? kTbLast : 0);
+      e.rd = b & 1;
+      e.tiles_x = (r->nx + S->W - 1) / S->W;
+      const int tiles = e.tiles_x * ((r->nz + S->H - 1) / S->H);
+      ents.push_back(e);
+      pres.push_back(pres.back() + tiles);
+      r->cursor++;
+      cu += (long long)e.nsub * r->cells();
+    }
+    const int n = (int)ents.size() - eo;
+    if (n == 0) {
+      pres.pop_back();
+      break;
+    }
+    lr.push_back({eo, po});
+    lnent.push_back(n);
+    ltiles.push_back(pres.back());
+  }
+  // elastic: per-step lists of running requests (ids are slots)
+  std::vector<int> elist;
+  std::vector<std::pair<int, int>> el;  // (offset, nact) per step
+  std::vector<int> etiles;
+  for (int q = 0; q < S->tick_steps && !act_e.empty(); ++q) {
+    const int g = S->g_e + q;
+    std::vector<int> ids;
+    std::vector<int> pre{0};
+    for (Req* r : act_e) {
+      const int m = g - r->e_start;
+      if (m < -1 || m >= 2 * r->nt) continue;
+      ids.push_back(r->slot);
+      pre.push_back(pre.back() + S->es[r->slot].tiles);
+      if (m >= 0) cu += r->cells();
+    }
+    el.push_back({(int)elist.size(), (int)ids.size()});
+    etiles.push_back(pre.back());
+    elist.insert(elist.end(), ids.begin(), ids.end());
+    elist.insert(elist.end(), pre.begin(), pre.end());
+  }
+  TSW_TRY(S->d_tbent.ensure(std::max<size_t>(ents.size(), 1) * sizeof(TbEntry)));
+  TSW_TRY(S->d_tbpre.ensure(std::max<size_t>(pres.size(), 1) * sizeof(int)));
+  TSW_TRY(S->d_elist.ensure(std::max<size_t>(elist.size(), 1) * sizeof(int)));
+  if (!ents.empty()) {
+    TSW_HIP(hipMemcpyAsync(S->d_tbent.p, ents.data(), ents.size() * sizeof(TbEntry),
+                           hipMemcpyHostToDevice, S->sa));
+    TSW_HIP(hipMemcpyAsync(S->d_tbpre.p, pres.data(), pres.size() * sizeof(int),
+                           hipMemcpyHostToDevice, S->sa));
+  }
+  if (!elist.empty())
+    TSW_HIP(hipMemcpyAsync(S->d_elist.p, elist.data(), elist.size() * sizeof(int),
+                           hipMemcpyHostToDevice, S->se));
+  const double t_start = S->now();
+  for (size_t q = 0; q < el.size(); ++q) {
+    if (el[q].second == 0) continue;
+    for (int pass = 0; pass < 2; ++pass)
+      ela_batch_launch(S->ec, (const EShot*)S->d_eshots.p, S->d_elist.i() + el[q].first,
+                       el[q].second, etiles[q], S->g_e + (int)q, pass, S->se);
+  }
+  for (size_t L = 0; L < lr.size(); ++L)
+    tb_launch(S->kernel, S->bc, (const TbShot*)S->d_tbshots.p,
+              (const TbEntry*)S->d_tbent.p + lr[L].first, S->d_tbpre.i() + lr[L].second,
+              lnent[L], ltiles[L], S->sa);
+  TSW_HIP(hipGetLastError());
+  // completion: acoustic after its last block, elastic after step 2*nt-1
+  const int g_end = S->g_e + S->tick_steps;
+  for (Req* r : act_e) {
+    if (g_end - r->e_start >= 2 * r->nt)
+      ela_scale_launch(r->illum.f(), r->cells(), S->ec.w, S->se);
+  }
+  TSW_HIP(hipStreamSynchronize(S->sa));
+  TSW_HIP(hipStreamSynchronize(S->se));
+  busy = S->now() - t_start;
+  if (!act_e.empty()) S->g_e = g_end;
+  for (Req* r : act_a) {
+    const int nb = (r->nt + S->T - 1) / S->T;
+    if (r->cursor >= 2 * nb) done.push_back(r);
+  }
+  for (Req* r : act_e)
+    if (S->g_e - r->e_start >= 2 * r->nt) done.push_back(r);
+  return kOk;
+}
+
+static void worker_loop(Server* S) {
+  std::unique_lock<std::mutex> lk(S->mu);
+  for (;;) {
+    // cancellations of running requests take effect between ticks
+    for (auto& sl : S->slots) {
+      Req* r = sl;
+      if (r && r->cancel) {
+        r->state = kCancelled;
+        r->t_done = S->now();
+        S->cancelled++;
+        S->mem_used -= r->need;
+        r->image.release();
+        r->illum.release();
+        release(S, r);
+      }
+    }
+    // admission, in arrival order, while a slot is free and the budget allows
+    while (!S->queue.empty()) {
+      Req* r = S->queue.front();
+      if (r->cancel) {
+        S->queue.pop_front();
+        continue;
+      }
+      int s = -1;
+      for (int j = 0; j < S->max_active; ++j)
+        if (!S->slots[j]) {
+          s = j;
+          break;
+        }
+      if (s < 0 || S->mem_used + r->need > S->budget) break;
+      S->queue.pop_front();
+      r->slot = s;
+      S->slots[s] = r;
+      r->state = kRunning;
+      r->t_admit = S->now();
+      S->mem_used += r->need;
+      const int rc = admit(S, r, s);
+      if (rc != kOk) {  // this request fails; the others are untouched
+        r->state = kFailed;
+        r->err = tsw_last_error();
+        r->t_done = S->now();
+        S->failed++;
+        S->mem_used -= r->need;
+        r->image.release();
+        r->illum.release();
+        release(S, r);
+      }
+    }
+    bool any = false;
+    for (Req* r : S->slots) any = any || r;
+    if (!any) {
+      if (S->stop) break;
+      S->cv.wait_for(lk, std::chrono::milliseconds(2));
+      continue;
+    }
+    std::vector<Req*> done;
+    long long cu = 0;
+    double busy = 0;
+    lk.unlock();
+    const int rc = tick(S, done, cu, busy);
+    lk.lock();
+    S->cell_updates += cu;
+    S->busy_s += busy;
+    const double t = S->now();
+    for (Req* r : done) {
+      r->state = kDone;
+      r->t_done = t;
+      S->completed++;
+      S->mem_used -= r->need - r->keep;
+      release(S, r);
+    }
+    S->cv.notify_all();
+    if (rc != kOk) {
+      S->fatal = tsw_last_error();
+      for (Req* r : S->slots)
+        if (r) {
+          r->state = kFailed;
+          r->err = S->fatal;
+          release(S, r);
+        }
+      break;
+    }
+  }
+  S->cv.notify_all();
+}
+
+}  // namespace tsw
+
+using namespace tsw;
+
+extern "C" {
+
+int tsw_server_create(double dx, double dz, double dt, int n_sponge, double sponge_alpha,
+                      int free_surface_top, int k, long long budget_bytes, int max_active,
+                      int kernel, int tick_steps, tsw_server_t* out) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  TSW_ARG(out, "out is null");
+  TSW_ARG(dx > 0 && dz > 0 && dt > 0 && k >= 1 && n_sponge >= 0, "bad constants");
+  TSW_ARG(budget_bytes > 0 && max_active >= 1 && tick_steps >= 1, "bad server limits");
+  int T, W, H;
+  TSW_ARG(tb_config(kernel, &T, &W, &H) == kOk, "kernel %d is not a blocked configuration",
+          kernel);
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  auto S = std::make_unique<Server>();
+  S->dt = dt;
+  S->k = k;
+  S->kernel = kernel;
+  S->T = T;
+  S->W = W;
+  S->H = H;
+  S->tick_steps = tick_steps;
+  S->max_active = max_active;
+  S->budget = budget_bytes;
+  S->t0 = std::chrono::steady_clock::now();
+  TSW_HIP(hipStreamCreateWithFlags(&S->sa, hipStreamNonBlocking));
+  TSW_HIP(hipStreamCreateWithFlags(&S->se, hipStreamNonBlocking));
+  std::vector<float> tab;
+  sponge_table(tab, n_sponge, (float)sponge_alpha);
+  TSW_TRY(alloc_up(S->d_table, tab.data(), tab.size() * sizeof(float), S->sa));
+  TSW_HIP(hipStreamSynchronize(S->sa));
+  const float dxf = (float)dx, dzf = (float)dz;
+  S->bc.fs = free_surface_top ? 1 : 0;
+  S->bc.n_sponge = n_sponge;
+  S->bc.rdx2 = 1.0f / (dxf * dxf);
+  S->bc.rdz2 = 1.0f / (dzf * dzf);
+  S->bc.inj = (float)(dt * dt);
+  S->bc.w = (float)(dt * k);
+  S->bc.table = S->d_table.f();
+  S->ec.fs = S->bc.fs;
+  S->ec.n_sponge = n_sponge;
+  S->ec.rdx = 1.0f / dxf;
+  S->ec.rdz = 1.0f / dzf;
+  S->ec.dt = (float)dt;
+  S->ec.w = (float)(dt * k);
+  S->ec.table = S->d_table.f();
+  S->slots.assign(max_active, nullptr);
+  S->tbs.resize(max_active);
+  S->es.resize(max_active);
+  TSW_TRY(S->d_tbshots.ensure(max_active * sizeof(TbShot)));
+  TSW_TRY(S->d_eshots.ensure(max_active * sizeof(EShot)));
+  Server* raw = S.release();
+  raw->worker = std::thread(worker_loop, raw);
+  *out = reinterpret_cast<tsw_server_t>(raw);
+  return kOk;
+}
+
+int tsw_server_submit(tsw_server_t h, const tsw_shot_desc* d, long long* id_out) {
+  Server* S = reinterpret_cast<Server*>(h);
+  TSW_ARG(S && d && id_out, "null argument");
+  TSW_ARG(d->kind == TSW_SHOT_ACOUSTIC || d->kind == TSW_SHOT_ELASTIC, "bad kind");
+  TSW_ARG(d->nx >= 5 && d->nz >= 5 && d->nt >= 1 && d->src, "bad request grid or source");
+  TSW_ARG(d->six >= 0 && d->six < d->nx && d->siz >= 0 && d->siz < d->nz, "source outside grid");
+  TSW_ARG(d->riz >= 0 && d->riz < d->nz && d->nrec >= 0, "bad receiver row");
+  TSW_ARG(d->nrec == 0 || (d->rix && d->data), "receivers need rix and data");
+  for (int r = 0; r < d->nrec; ++r)
+    TSW_ARG(d->rix[r] >= 0 && d->rix[r] < d->nx, "receiver %d outside grid", r);
+  auto r = std::make_unique<Req>();
+  r->kind = d->kind;
+  r->nx = d->nx;
+  r->nz = d->nz;
+  r->nt = d->nt;
+  r->six = d->six;
+  r->siz = d->siz;
+  r->riz = d->riz;
+  r->nrec = d->nrec;
+  const size_t n = (size_t)d->nx * d->nz;
+  if (d->kind == TSW_SHOT_ACOUSTIC) {
+    TSW_ARG(d->vel_fwd && d->vel_bwd, "missing velocity models");
+    r->model[0].assign(d->vel_fwd, d->vel_fwd + n);
+    r->model[1].assign(d->vel_bwd, d->vel_bwd + n);
+  } else {
+    for (int j = 0; j < 4; ++j) {
+      TSW_ARG(d->ela_fwd[j] && d->ela_bwd[j], "missing elastic models");
+      r->model[j].assign(d->ela_fwd[j], d->ela_fwd[j] + n);
+      r->model[4 + j].assign(d->ela_bwd[j], d->ela_bwd[j] + n);
+    }
+  }
+  r->src.assign(d->src, d->src + d->nt);
+  if (d->nrec) r->data.assign(d->data, d->data + (size_t)d->nt * d->nrec);
+  r->rix.assign(d->rix, d->rix + d->nrec);
+  std::lock_guard<std::mutex> lk(S->mu);
+  TSW_ARG(!S->stop, "server is shutting down");
+  r->need = req_need(*r, S->k);
+  r->keep = 2 * 4 * (long long)n;
+  if (r->need > S->budget) {
+    set_error("request needs %lld bytes, more than the %lld-byte budget", r->need, S->budget);
+    return kErrMem;
+  }
+  r->id = S->next_id++;
+  r->t_submit = S->now();
+  *id_out = r->id;
+  S->queue.push_back(r.get());
+  S->reqs[r->id] = std::move(r);
+  S->cv.notify_all();
+  return kOk;
+}
+
+int tsw_server_poll(tsw_server_t h, long long id, int* state, double* t_submit, double* t_admit,
+                    double* t_done) {
+  Server* S = reinterpret_cast<Server*>(h);
+  TSW_ARG(S && state, "null argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(id);
+  TSW_ARG(it != S->reqs.end(), "unknown request %lld", id);
+  const Req& r = *it->second;
+  *state = r.state;
+  if (t_submit) *t_submit = r.t_submit;
+  if (t_admit) *t_admit = r.t_admit;
+  if (t_done) *t_done = r.t_done;
+  if (r.state == kFailed) set_error("%s", r.err.c_str());
+  return kOk;
+}
+
+int tsw_server_wait(tsw_server_t h, long long id, double timeout_s) {
+  Server* S = reinterpret_cast<Server*>(h);
+  TSW_ARG(S, "null server");
+  std::unique_lock<std::mutex> lk(S->mu);
+  const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+  for (;;) {
+    auto it = S->reqs.find(id);
+    TSW_ARG(it != S->reqs.end(), "unknown request %lld", id);
+    if (it->second->state >= kDone) return kOk;
+    if (!S->fatal.empty()) {
+      set_error("server stopped: %s", S->fatal.c_str());
+      return kErrState;
+    }
+    if (S->cv.wait_until(lk, until) == std::cv_status::timeout) {
+      set_error("request %lld not finished after %.1f s", id, timeout_s);
+      return kErrState;
+    }
+  }
+}
+
+int tsw_server_result(tsw_server_t h, long long id, float* image_out, float* illum_out) {
+  Server* S = reinterpret_cast<Server*>(h);
+  TSW_ARG(S, "null server");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(id);
+  TSW_ARG(it != S->reqs.end(), "unknown request %lld", id);
+  Req& r = *it->second;
+  if (r.state == kFailed) {
+    set_error("request %lld failed: %s", id, r.err.c_str());
+    return kErrState;
+  }
+  TSW_ARG(r.state == kDone, "request %lld is not done (state %d)", id, r.state);
+  const size_t nb = (size_t)r.cells() * sizeof(float);
+  if (image_out) TSW_HIP(hipMemcpy(image_out, r.image.p, nb, hipMemcpyDeviceToHost));
+  if (illum_out) TSW_HIP(hipMemcpy(illum_out, r.illum.p, nb, hipMemcpyDeviceToHost));
+  r.image.release();
+  r.illum.release();
+  S->mem_used -= r.keep;
+  S->reqs.erase(it);
+  return kOk;
+}
+
+int tsw_server_cancel(tsw_server_t h, long long id) {
+  Server* S = reinterpret_cast<Server*>(h);
+  TSW_ARG(S, "null server");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(id);
+  TSW_ARG(it != S->reqs.end(), "unknown request %lld", id);
+  Req& r = *it->second;
+  if (r.state == kQueued) {
+    r.cancel = true;
+    r.state = kCancelled;
+    S->cancelled++;
+  } else if (r.state == kRunning) {
+    r.cancel = true;  // takes effect at the next tick boundary
+  }
+  S->cv.notify_all();
+  return kOk;
+}
+
+int tsw_server_stats(tsw_server_t h, long long* completed, long long* failed,
+                     long long* cancelled, long long* queued, long long* running,
+                     long long* mem_used, long long* cell_updates, double* busy_s) {
+  Server* S = reinterpret_cast<Server*>(h);
+  TSW_ARG(S, "null server");
+  std::lock_guard<std::mutex> lk(S->mu);
+  long long q = 0, run = 0;
+  for (Req* r : S->queue) q += r->cancel ? 0 : 1;
+  for (Req* r : S->slots) run += r ? 1 : 0;
+  if (completed) *completed = S->completed;
+  if (failed) *failed = S->failed;
+  if (cancelled) *cancelled = S->cancelled;
+  if (queued) *queued = q;
+  if (running) *running = run;
+  if (mem_used) *mem_used = S->mem_used;
+  if (cell_updates) *cell_updates = S->cell_updates;
+  if (busy_s) *busy_s = S->busy_s;
+  return kOk;
+}
+
+int tsw_server_destroy(tsw_server_t h) {
+  Server* S = reinterpret_cast<Server*>(h);
+  TSW_ARG(S, "null server");
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+    for (Req* r : S->queue) r->cancel = true;
+    for (Req* r : S->slots)
+      if (r) r->cancel = true;
+    S->cv.notify_all();
+  }
+  if (S->worker.joinable()) S->worker.join();
+  (void)hipStreamSynchronize(S->sa);
+  (void)hipStreamSynchronize(S->se);
+  (void)hipStreamDestroy(S->sa);
+  (void)hipStreamDestroy(S->se);
+  delete S;
+  return kOk;
+}
+
+}  // extern "C"

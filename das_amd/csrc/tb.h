@@ -63,6 +63,12 @@ __device__ __forceinline__ int xcd_tile_tb(int b, int n) {
   return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
 }
 
+// Configuration `id` (1-based) of the instantiated blocked kernels: steps per
+// launch and core tile; and a launch of it (defined in batch.hip).
+int tb_config(int id, int* T, int* W, int* H);
+void tb_launch(int id, const BatchConst& bc, const TbShot* shots, const TbEntry* ents,
+               const int* pre, int nent, int ntiles, hipStream_t st);
+
 template <int T, int R, int WX, int WZ>
 struct TbShape {
   static constexpr int CX = 64 * WX;     // extended columns

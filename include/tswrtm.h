@@ -29,6 +29,7 @@ extern "C" {
 
 typedef struct tsw_grid_s* tsw_grid_t;
 typedef struct tsw_batch_s* tsw_batch_t;
+typedef struct tsw_server_s* tsw_server_t;
 
 /* One virtual shot of a batch, on its own (nz, nx) grid.  The shot's grid is
  * the window [z0, z0+nz) x [x0, x0+nx) of a larger stacking grid. */
@@ -199,6 +200,34 @@ int tsw_batch_stack(tsw_batch_t batch, int gnx, int gnz, float* image_out, float
 int tsw_batch_info(tsw_batch_t batch, long long* cell_updates, long long* device_bytes,
                    int* total_steps, int* n_graphs, long long* elastic_cell_updates);
 int tsw_batch_destroy(tsw_batch_t batch);
+
+/* Serving loop (no reference counterpart; the reference migrates a fixed
+ * list of shots in a Python loop, rtm_engine.py:125-143).  A worker thread
+ * admits submitted requests (acoustic or elastic shots, tsw_shot_desc) in
+ * arrival order while fewer than `max_active` run and their HBM footprint fits
+ * `budget_bytes`, and advances all running requests together `tick_steps`
+ * steps at a time (blocked acoustic kernel configuration `kernel` on one
+ * stream, elastic steps on another).  Results are bit-identical to migrating
+ * each request alone.  Times are seconds on a monotonic clock since creation.
+ * Request states: 0 queued, 1 running, 2 done, 3 failed, 4 cancelled. */
+int tsw_server_create(double dx, double dz, double dt, int n_sponge, double sponge_alpha,
+                      int free_surface_top, int k, long long budget_bytes, int max_active,
+                      int kernel, int tick_steps, tsw_server_t* out);
+/* Copies the request's host arrays and queues it; -3 if it can never fit. */
+int tsw_server_submit(tsw_server_t server, const tsw_shot_desc* req, long long* id_out);
+int tsw_server_poll(tsw_server_t server, long long id, int* state, double* t_submit,
+                    double* t_admit, double* t_done);
+/* Blocks until request `id` is done, failed or cancelled (or the timeout). */
+int tsw_server_wait(tsw_server_t server, long long id, double timeout_s);
+/* Copies a done request's image and illumination and forgets the request. */
+int tsw_server_result(tsw_server_t server, long long id, float* image_out, float* illum_out);
+/* Cancels a queued request, or a running one at the next tick boundary. */
+int tsw_server_cancel(tsw_server_t server, long long id);
+int tsw_server_stats(tsw_server_t server, long long* completed, long long* failed,
+                     long long* cancelled, long long* queued, long long* running,
+                     long long* mem_used, long long* cell_updates, double* busy_s);
+/* Cancels everything outstanding, stops the worker and frees the server. */
+int tsw_server_destroy(tsw_server_t server);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,147 @@
+"""Serving loop: each request's result equals migrating it alone (bit for
+bit), whether it runs alone or in a mix of uneven acoustic and elastic requests
+that queue against a tight memory budget; admission respects the slot and
+budget limits; cancellation and an injected admission failure affect only the
+request concerned."""
+import time
+
+import numpy as np
+import pytest
+
+from das_amd import _native as N
+
+pytestmark = pytest.mark.gpu
+
+DX, DT, NSP, K = 5.0, 4e-4, 8, 2
+
+
+def _requests(seed=3):
+    from das_amd.wave_solver import WaveSolver2D, ricker_wavelet
+    from das_amd.workloads import (ElasticShot, Shot, correlogram_gather, elastic_from_vp,
+                                   faulted_two_layer, layered_faulted_vp, smooth)
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    specs = [("a", 64, 200, 130), ("e", 40, 96, 70), ("a", 96, 160, 90), ("a", 48, 256, 151),
+             ("e", 56, 120, 61), ("a", 70, 300, 77)]
+    for kind, nz, nx, nt in specs:
+        six = int(rng.integers(5, nx - 5))
+        rix = np.arange(1, nx - 1, 2).astype(np.int32)
+        src, _ = ricker_wavelet(25.0, DT, nt * DT)
+        if kind == "a":
+            v = smooth(faulted_two_layer(nz, nx, DX, seed=int(rng.integers(99))), 30.0, DX)
+            data = correlogram_gather(rng, nt, DT, (rix - six) * DX, f0=25.0, f_lo=5, f_hi=50)
+            reqs.append(Shot(nx=nx, nz=nz, nt=nt, vel_fwd=v, vel_bwd=v, src=src, six=six,
+                             siz=2, data=data, rix=rix, riz=2))
+        else:
+            vp = layered_faulted_vp(nz, nx, DX, seed=int(rng.integers(99)))
+            mt, mm = elastic_from_vp(vp), elastic_from_vp(smooth(vp, 30.0, DX))
+            solver = WaveSolver2D(nx, nz, DX, DX, DT, n_sponge=NSP, solver_type="elastic")
+            data, _ = solver.forward_propagate_elastic(mt, src, six, 2, rix, 2)
+            reqs.append(ElasticShot(nx=nx, nz=nz, nt=nt, model_fwd=mm, model_bwd=mm, src=src,
+                                    six=six, siz=2, data=data, rix=rix, riz=2))
+    return reqs
+
+
+def _alone(s):
+    from das_amd.rtm_engine import TSWRTMEngine
+    from das_amd.velocity_flood import FloodModels
+    from das_amd.wave_solver import WaveSolver2D
+
+    solver = WaveSolver2D(s.nx, s.nz, DX, DX, DT, n_sponge=NSP)
+    eng = TSWRTMEngine(solver)
+    if getattr(s, "kind", "acoustic") == "elastic":
+        return eng.migrate_shot_elastic(s.model_fwd, s.model_bwd, s.src, s.six, s.siz, s.data,
+                                        s.rix, s.riz, K)
+    return eng.migrate_shot(FloodModels(s.vel_fwd, s.vel_fwd, s.vel_bwd, 0, 0), s.src, s.six,
+                            s.siz, s.data, s.rix, s.riz, K)
+
+
+@pytest.fixture(scope="module")
+def reqs():
+    N.check(N.lib().tsw_init(0))
+    r = _requests()
+    return r, [_alone(s) for s in r]
+
+
+def _server(**kw):
+    from das_amd.serve import RTMServer
+
+    args = dict(n_sponge=NSP, k=K, budget_bytes=4e9, max_active=8, kernel=11, tick_steps=40)
+    args.update(kw)
+    return RTMServer(DX, DX, DT, **args)
+
+
+def test_each_request_alone_equals_single_shot(reqs):
+    shots, expect = reqs
+    srv = _server()
+    for s, (img, ill) in zip(shots, expect):
+        rid = srv.submit(s)
+        assert srv.wait(rid)["state"] == "done"
+        got = srv.result(rid)
+        np.testing.assert_array_equal(got[0], img)
+        np.testing.assert_array_equal(got[1], ill)
+    srv.close()
+
+
+def test_mix_under_tight_budget_equals_alone(reqs):
+    from das_amd.autotune import free_device_bytes  # noqa: F401  (device query works)
+
+    shots, expect = reqs
+    need = []
+    for s in shots:
+        n = s.nx * s.nz
+        per = (10 if getattr(s, "kind", "") != "elastic" else 20) * 4 * n
+        need.append(per + 4 * ((s.nt - 1) // K + 1) * n)
+    budget = int(2.2 * max(need))  # forces queueing: at most a few requests fit at once
+    srv = _server(budget_bytes=budget, max_active=3, tick_steps=17)
+    ids = []
+    for i, s in enumerate(shots):
+        ids.append(srv.submit(s))
+        time.sleep(0.002 * (i % 3))  # staggered arrivals while others run
+    polls = [srv.wait(r) for r in ids]
+    assert all(p["state"] == "done" for p in polls)
+    for rid, (img, ill) in zip(ids, expect):
+        got = srv.result(rid)
+        np.testing.assert_array_equal(got[0], img)
+        np.testing.assert_array_equal(got[1], ill)
+    # admission order = arrival order, and never more than 3 running at once
+    admits = sorted(p["t_admit"] for p in polls)
+    assert [p["t_admit"] for p in polls] == admits
+    events = sorted([(p["t_admit"], 1) for p in polls] + [(p["t_done"], -1) for p in polls])
+    run = peak = 0
+    for _, d in events:
+        run += d
+        peak = max(peak, run)
+    assert peak <= 3
+    st = srv.stats()
+    assert st["completed"] == len(shots) and st["failed"] == 0 and st["mem_used"] == 0
+    srv.close()
+
+
+def test_budget_rejection_cancel_and_injected_failure(reqs):
+    shots, expect = reqs
+    srv = _server(budget_bytes=1e6)
+    with pytest.raises(MemoryError):
+        srv.submit(shots[0])
+    srv.close()
+    srv = _server(max_active=1)
+    a = srv.submit(shots[3])
+    b = srv.submit(shots[0])
+    srv.cancel(b)  # still queued behind a
+    assert srv.wait(a)["state"] == "done"
+    assert srv.poll(b)["state"] == "cancelled"
+    np.testing.assert_array_equal(srv.result(a)[0], expect[3][0])
+    N.lib().tsw_debug_fail_next(1)
+    c = srv.submit(shots[2])
+    d = srv.submit(shots[1])
+    assert srv.wait(c)["state"] == "failed"
+    assert srv.wait(d)["state"] == "done"
+    got = srv.result(d)
+    np.testing.assert_array_equal(got[0], expect[1][0])
+    np.testing.assert_array_equal(got[1], expect[1][1])
+    with pytest.raises(N.TswError):
+        srv.result(c)
+    st = srv.stats()
+    assert st["failed"] == 1 and st["cancelled"] == 1
+    srv.close()
