@@ -15,6 +15,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -100,6 +102,42 @@ def elastic_leg(args, log):
                          "algorithmic_bytes_per_cell_update": ELASTIC_BYTES_PER_CELL_UPDATE}}
 
 
+def served_leg(args, acoustic_gcell, elastic_gcell, kernel, log):
+    """Served RTM: a seeded Poisson stream of virtual-shot requests (nz in
+    {256, 384, 512} x nx in {512, 1024, 2048}, nt ~ U{2000..8000}, 80% acoustic /
+    20% elastic) against a 200 GB HBM budget.  The arrival rate is stated as a
+    fraction of the capacity implied by the batched acoustic and elastic
+    throughputs measured above."""
+    from das_amd.serve import RTMServer, run_poisson
+    from das_amd.workloads import Config4
+
+    cfg = Config4(args.served_requests)
+    t_req = np.mean([2 * s.nt * s.cells / 1e9 /
+                     (elastic_gcell if s.kind == "elastic" else acoustic_gcell)
+                     for s in cfg.specs])
+    capacity = 1.0 / t_req
+    rate = args.served_load * capacity
+    log(f"served: {len(cfg.specs)} requests at {rate:.2f}/s "
+        f"({args.served_load} x capacity {capacity:.2f}/s)")
+    srv = RTMServer(cfg.dx, cfg.dx, cfg.dt, n_sponge=cfg.n_sponge, k=cfg.k, budget_bytes=200e9,
+                    max_active=args.served_slots, kernel=kernel, tick_steps=100)
+    r = run_poisson(srv, cfg.specs, rate, seed=11, make=cfg.make, log=log)
+    srv.close()
+    n_el = sum(s.kind == "elastic" for s in cfg.specs)
+    return {"workload": f"{len(cfg.specs)} Poisson-arrival virtual-shot migrations, "
+                        f"{len(cfg.specs) - n_el} acoustic / {n_el} elastic, nz {{256,384,512}} "
+                        "x nx {512,1024,2048}, nt 2000-8000, dx 5 m, dt 0.4 ms, 200 GB budget",
+            "arrival_rate_per_s": round(rate, 3), "load_vs_capacity": args.served_load,
+            "capacity_estimate_per_s": round(capacity, 3),
+            "p50_latency_s": round(r["p50_latency_s"], 4),
+            "p99_latency_s": round(r["p99_latency_s"], 4),
+            "shots_per_s": round(r["shots_per_s"], 3),
+            "gcell_updates_per_s": round(r["gcell_updates_per_s"], 2),
+            "failed": r["failed"], "span_s": round(r["span_s"], 2),
+            "gpu_busy_s": round(r["gpu_busy_s"], 2), "max_active": args.served_slots,
+            "kernel": kernel}
+
+
 def traffic_from_profiles():
     """HBM bytes per launch of the step kernel from the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
@@ -124,6 +162,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-elastic", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-requests", type=int, default=400)
+    ap.add_argument("--served-load", type=float, default=0.8)
+    ap.add_argument("--served-slots", type=int, default=32)
     ap.add_argument("--cpu-budget", type=float, default=2.5, help="G cell-updates")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -210,6 +252,9 @@ def main():
     batch.close()
     if not args.no_elastic:
         out["elastic"] = elastic_leg(args, log)
+    if not args.no_served:
+        el = out.get("elastic", {}).get("value", 50.0)
+        out["served"] = served_leg(args, value, el, args.kernel, log)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget * 1e9)
     print(json.dumps(out), flush=True)

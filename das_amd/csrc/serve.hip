@@ -32,6 +32,51 @@ namespace tsw {
 
 enum ReqState { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3, kCancelled = 4 };
 
+// First-fit sub-allocator over one device allocation made at server start:
+// per-request hipMalloc / hipFree of multi-GB snapshot arenas would stall the
+// serving loop for milliseconds each.  Free blocks are coalesced on release.
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_;  // offset -> bytes
+  static size_t up(size_t b) { return (b + 255) & ~(size_t)255; }
+  bool fits(size_t bytes) const {
+    bytes = up(bytes);
+    for (const auto& f : free_)
+      if (f.second >= bytes) return true;
+    return false;
+  }
+  char* alloc(size_t bytes) {
+    bytes = up(bytes);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, rest = it->second - bytes;
+      free_.erase(it);
+      if (rest) free_[off + bytes] = rest;
+      return base + off;
+    }
+    return nullptr;
+  }
+  void release(char* p, size_t bytes) {
+    if (!p) return;
+    bytes = up(bytes);
+    size_t off = (size_t)(p - base);
+    auto nx = free_.lower_bound(off);
+    if (nx != free_.end() && off + bytes == nx->first) {  // merge with the next block
+      bytes += nx->second;
+      nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {  // merge with the previous block
+      auto pv = std::prev(nx);
+      if (pv->first + pv->second == off) {
+        pv->second += bytes;
+        return;
+      }
+    }
+    free_[off] = bytes;
+  }
+};
+
 struct Req {
   long long id = 0;
   int kind = TSW_SHOT_ACOUSTIC;
@@ -47,8 +92,11 @@ struct Req {
   int slot = -1;
   int cursor = 0;         // acoustic: next block; elastic: unused (global step based)
   int e_start = 0;        // elastic: global elastic step of the first forward step
-  std::vector<DevBuf> heavy;   // released at completion
-  DevBuf image, illum;         // released when fetched
+  char* heavy = nullptr;       // arena block: state, models, data, snapshots
+  size_t heavy_bytes = 0;      //   (released at completion)
+  char* keepb = nullptr;       // arena block: image + illumination (released when fetched)
+  float* image = nullptr;
+  float* illum = nullptr;
   std::string err;
   long long cells() const { return (long long)nx * nz; }
 };
@@ -62,6 +110,8 @@ struct Server {
   EConst ec{};
   hipStream_t sa = nullptr, se = nullptr;
   DevBuf d_table, d_tbshots, d_eshots, d_tbent, d_tbpre, d_elist;
+  Arena arena;
+  void* arena_mem = nullptr;
   std::vector<TbShot> tbs;
   std::vector<EShot> es;
   // state
@@ -84,14 +134,20 @@ struct Server {
   }
 };
 
-static long long req_need(const Req& r, int k) {
+// Bytes of a request's heavy block (the layout admit() carves) and of its
+// kept block (image + illumination).
+static long long req_heavy(const Req& r, int k) {
+  auto up = [](long long b) { return (long long)Arena::up((size_t)(b ? b : 4)); };
   const long long n = r.cells();
+  const long long nb = up(4 * n);
   const long long nsnap = (r.nt - 1) / k + 1;
-  const long long io = 4LL * (r.nt + (long long)r.nt * std::max(r.nrec, 1)) + 4LL * (r.nx + 1 + r.nrec);
-  // models, wavefields, snapshot arena, data; image + illumination (2 n) included
-  if (r.kind == TSW_SHOT_ACOUSTIC) return 4 * n * (2 + 6 + 2) + 4 * nsnap * n + io;
-  return 4 * n * (8 + 10 + 2) + 4 * nsnap * n + io;
+  long long b = up(4LL * r.nt) + up(4LL * r.nt * r.nrec) + up(4LL * (r.nx + 1)) +
+                up(4LL * std::max(r.nrec, 1)) + up(4 * nsnap * n);
+  if (r.kind == TSW_SHOT_ACOUSTIC) b += 2 * nb + nb /* staging */ + 6 * nb;
+  else b += 8 * up(4 * n) + 10 * nb;
+  return b;
 }
+static long long req_keep(const Req& r) { return 2 * (long long)Arena::up((size_t)(4 * r.cells())); }
 
 static int alloc_up(DevBuf& d, const void* h, size_t bytes, hipStream_t st) {
   TSW_TRY(d.ensure(bytes ? bytes : 4));
@@ -99,38 +155,49 @@ static int alloc_up(DevBuf& d, const void* h, size_t bytes, hipStream_t st) {
   return kOk;
 }
 
-// Allocates and uploads a request's device state into slot `s`.
+// Uploads a request's inputs into its arena blocks (already reserved) and
+// writes its descriptor into slot `s`.  Called by the worker without the lock.
 static int admit(Server* S, Req* r, int s) {
   TSW_TRY(fail_injected());
   hipStream_t st = r->kind == TSW_SHOT_ACOUSTIC ? S->sa : S->se;
   const long long n = r->cells();
-  const size_t nb = (size_t)n * sizeof(float);
+  const size_t nb = Arena::up((size_t)n * sizeof(float));
   const long long nsnap = (r->nt - 1) / S->k + 1;
   RecvMap m;
   TSW_TRY(m.build(r->rix.data(), r->nrec, r->nx));
-  r->heavy.clear();
-  r->heavy.resize(r->kind == TSW_SHOT_ACOUSTIC ? 13 : 23);
-  auto& H = r->heavy;
-  TSW_TRY(alloc_up(H[0], r->src.data(), r->src.size() * sizeof(float), st));
-  TSW_TRY(alloc_up(H[1], r->data.data(), r->data.size() * sizeof(float), st));
-  TSW_TRY(alloc_up(H[2], m.off.data(), m.off.size() * sizeof(int), st));
-  TSW_TRY(alloc_up(H[3], m.idx.data(), m.idx.size() * sizeof(int), st));
-  TSW_TRY(r->image.ensure(nb));
-  TSW_TRY(r->illum.ensure(nb));
+  char* cur = r->heavy;
+  auto take = [&](size_t bytes) {
+    char* p = cur;
+    cur += Arena::up(bytes);
+    return p;
+  };
+  auto up = [&](const void* h, size_t bytes) -> char* {
+    char* p = take(bytes ? bytes : 4);
+    if (bytes) (void)hipMemcpyAsync(p, h, bytes, hipMemcpyHostToDevice, st);
+    return p;
+  };
+  float* src = (float*)up(r->src.data(), r->src.size() * sizeof(float));
+  float* data = (float*)up(r->data.data(), r->data.size() * sizeof(float));
+  int* roff = (int*)up(m.off.data(), m.off.size() * sizeof(int));
+  int* ridx = (int*)up(m.idx.data(), m.idx.size() * sizeof(int));
+  r->image = (float*)r->keepb;
+  r->illum = (float*)(r->keepb + nb);
   if (r->kind == TSW_SHOT_ACOUSTIC) {
-    // [4],[5] coefficients; [6..11] two sets of three levels; [12] snapshots
-    DevBuf vel;
-    for (int j = 0; j < 2; ++j) {
-      TSW_TRY(alloc_up(vel, r->model[j].data(), nb, st));
-      TSW_TRY(H[4 + j].ensure(nb));
-      const float dtf = (float)S->dt;
-      launch_coef(vel.f(), H[4 + j].f(), n, dtf * dtf, st);
-      TSW_HIP(hipStreamSynchronize(st));  // `vel` is reused / freed
-    }
-    for (int j = 6; j < 12; ++j) TSW_TRY(H[j].ensure(nb));
-    TSW_TRY(H[12].ensure((size_t)nsnap * nb));
     TbShot P;
     std::memset(&P, 0, sizeof P);
+    const float dtf = (float)S->dt;
+    float* coef[2];
+    for (int j = 0; j < 2; ++j) {
+      coef[j] = (float*)take(nb);
+      float* vel = (float*)up(r->model[j].data(), (size_t)n * sizeof(float));
+      launch_coef(vel, coef[j], n, dtf * dtf, st);  // `vel` space is reused below
+      cur = (char*)vel;
+    }
+    for (int j = 0; j < 3; ++j) {
+      P.set[0][j] = (float*)take(nb);
+      P.set[1][j] = (float*)take(nb);
+    }
+    P.snaps = (float*)take((size_t)nsnap * n * sizeof(float));
     P.nx = r->nx;
     P.nz = r->nz;
     P.nt = r->nt;
@@ -138,30 +205,30 @@ static int admit(Server* S, Req* r, int s) {
     P.src_cell = r->siz * r->nx + r->six;
     P.riz = r->riz;
     P.nrec = r->nrec;
-    for (int j = 0; j < 3; ++j) {
-      P.set[0][j] = H[6 + j].f();
-      P.set[1][j] = H[9 + j].f();
-    }
-    P.coef_f = H[4].f();
-    P.coef_b = H[5].f();
-    P.src = H[0].f();
-    P.data = H[1].f();
-    P.roff = H[2].i();
-    P.ridx = H[3].i();
-    P.snaps = H[12].f();
-    P.illum = r->illum.f();
-    P.image = r->image.f();
+    P.coef_f = coef[0];
+    P.coef_b = coef[1];
+    P.src = src;
+    P.data = data;
+    P.roff = roff;
+    P.ridx = ridx;
+    P.illum = r->illum;
+    P.image = r->image;
     S->tbs[s] = P;
     TSW_HIP(hipMemcpyAsync((TbShot*)S->d_tbshots.p + s, &S->tbs[s], sizeof(TbShot),
                            hipMemcpyHostToDevice, st));
     r->cursor = 0;
   } else {
-    // [4..11] models (4 fwd, 4 bwd); [12..21] fields; [22] snapshots
-    for (int j = 0; j < 8; ++j) TSW_TRY(alloc_up(H[4 + j], r->model[j].data(), nb, st));
-    for (int j = 12; j < 22; ++j) TSW_TRY(H[j].ensure(nb));
-    TSW_TRY(H[22].ensure((size_t)nsnap * nb));
     EShot E;
     std::memset(&E, 0, sizeof E);
+    for (int j = 0; j < 4; ++j) {
+      E.mf[j] = (float*)up(r->model[j].data(), (size_t)n * sizeof(float));
+      E.mb[j] = (float*)up(r->model[4 + j].data(), (size_t)n * sizeof(float));
+    }
+    for (int j = 0; j < 5; ++j) {
+      E.f[j] = (float*)take(nb);
+      E.fb[j] = (float*)take(nb);
+    }
+    E.snaps = (float*)take((size_t)nsnap * n * sizeof(float));
     E.nx = r->nx;
     E.nz = r->nz;
     E.nt = r->nt;
@@ -170,35 +237,41 @@ static int admit(Server* S, Req* r, int s) {
     E.src_cell = r->siz * r->nx + r->six;
     E.riz = r->riz;
     E.nrec = r->nrec;
-    for (int j = 0; j < 5; ++j) {
-      E.f[j] = H[12 + j].f();
-      E.fb[j] = H[17 + j].f();
-    }
-    for (int j = 0; j < 4; ++j) {
-      E.mf[j] = H[4 + j].f();
-      E.mb[j] = H[8 + j].f();
-    }
-    E.src = H[0].f();
-    E.data = H[1].f();
-    E.roff = H[2].i();
-    E.ridx = H[3].i();
-    E.snaps = H[22].f();
-    E.illum = r->illum.f();
-    E.image = r->image.f();
+    E.src = src;
+    E.data = data;
+    E.roff = roff;
+    E.ridx = ridx;
+    E.illum = r->illum;
+    E.image = r->image;
     E.tiles = ela_tiles(r->nx, r->nz, &E.tiles_x);
     S->es[s] = E;
     TSW_HIP(hipMemcpyAsync((EShot*)S->d_eshots.p + s, &S->es[s], sizeof(EShot),
                            hipMemcpyHostToDevice, st));
     r->e_start = E.start;
   }
+  TSW_HIP(hipGetLastError());
   TSW_HIP(hipStreamSynchronize(st));
+  if ((size_t)(cur - r->heavy) > r->heavy_bytes) {
+    set_error("internal: request layout exceeds its reservation");
+    return kErrState;
+  }
+  // the host copies are no longer needed once the request lives in HBM
+  for (auto& v : r->model) std::vector<float>().swap(v);
+  std::vector<float>().swap(r->src);
+  std::vector<float>().swap(r->data);
   return kOk;
 }
 
 static void release(Server* S, Req* r) {
-  r->heavy.clear();
+  S->arena.release(r->heavy, r->heavy_bytes);
+  r->heavy = nullptr;
   if (r->slot >= 0) S->slots[r->slot] = nullptr;
   r->slot = -1;
+}
+static void release_keep(Server* S, Req* r) {
+  S->arena.release(r->keepb, (size_t)r->keep);
+  r->keepb = nullptr;
+  r->image = r->illum = nullptr;
 }
 
 // One tick: tick_steps time steps of every running request.  Runs without
@@ -295,7 +368,7 @@ static int tick(Server* S, std::vector<Req*>& done, long long& cu, double& busy)
   const int g_end = S->g_e + S->tick_steps;
   for (Req* r : act_e) {
     if (g_end - r->e_start >= 2 * r->nt)
-      ela_scale_launch(r->illum.f(), r->cells(), S->ec.w, S->se);
+      ela_scale_launch(r->illum, r->cells(), S->ec.w, S->se);
   }
   TSW_HIP(hipStreamSynchronize(S->sa));
   TSW_HIP(hipStreamSynchronize(S->se));
@@ -310,23 +383,25 @@ static int tick(Server* S, std::vector<Req*>& done, long long& cu, double& busy)
   return kOk;
 }
 
+static void fail_request(Server* S, Req* r, int state, const char* err) {
+  r->state = state;
+  if (err) r->err = err;
+  r->t_done = S->now();
+  if (state == kFailed) S->failed++;
+  else S->cancelled++;
+  S->mem_used -= r->heavy_bytes + r->keep;
+  release(S, r);
+  release_keep(S, r);
+}
+
 static void worker_loop(Server* S) {
   std::unique_lock<std::mutex> lk(S->mu);
   for (;;) {
     // cancellations of running requests take effect between ticks
-    for (auto& sl : S->slots) {
-      Req* r = sl;
-      if (r && r->cancel) {
-        r->state = kCancelled;
-        r->t_done = S->now();
-        S->cancelled++;
-        S->mem_used -= r->need;
-        r->image.release();
-        r->illum.release();
-        release(S, r);
-      }
-    }
-    // admission, in arrival order, while a slot is free and the budget allows
+    for (Req* r : S->slots)
+      if (r && r->cancel) fail_request(S, r, kCancelled, nullptr);
+    // admission, in arrival order, while a slot is free and the arena has room
+    std::vector<Req*> admitted;
     while (!S->queue.empty()) {
       Req* r = S->queue.front();
       if (r->cancel) {
@@ -339,24 +414,34 @@ static void worker_loop(Server* S) {
           s = j;
           break;
         }
-      if (s < 0 || S->mem_used + r->need > S->budget) break;
+      if (s < 0) break;
+      char* h = S->arena.alloc(r->heavy_bytes);
+      char* kb = h ? S->arena.alloc((size_t)r->keep) : nullptr;
+      if (!kb) {
+        S->arena.release(h, r->heavy_bytes);
+        break;
+      }
       S->queue.pop_front();
+      r->heavy = h;
+      r->keepb = kb;
       r->slot = s;
       S->slots[s] = r;
       r->state = kRunning;
       r->t_admit = S->now();
-      S->mem_used += r->need;
-      const int rc = admit(S, r, s);
-      if (rc != kOk) {  // this request fails; the others are untouched
-        r->state = kFailed;
-        r->err = tsw_last_error();
-        r->t_done = S->now();
-        S->failed++;
-        S->mem_used -= r->need;
-        r->image.release();
-        r->illum.release();
-        release(S, r);
+      S->mem_used += r->heavy_bytes + r->keep;
+      admitted.push_back(r);
+    }
+    if (!admitted.empty()) {
+      lk.unlock();
+      std::vector<int> rcs;
+      std::vector<std::string> errs;
+      for (Req* r : admitted) {
+        rcs.push_back(admit(S, r, r->slot));
+        errs.push_back(rcs.back() == kOk ? std::string() : std::string(tsw_last_error()));
       }
+      lk.lock();
+      for (size_t j = 0; j < admitted.size(); ++j)  // a failed admission fails that request only
+        if (rcs[j] != kOk) fail_request(S, admitted[j], kFailed, errs[j].c_str());
     }
     bool any = false;
     for (Req* r : S->slots) any = any || r;
@@ -378,18 +463,14 @@ static void worker_loop(Server* S) {
       r->state = kDone;
       r->t_done = t;
       S->completed++;
-      S->mem_used -= r->need - r->keep;
+      S->mem_used -= r->heavy_bytes;
       release(S, r);
     }
     S->cv.notify_all();
     if (rc != kOk) {
       S->fatal = tsw_last_error();
       for (Req* r : S->slots)
-        if (r) {
-          r->state = kFailed;
-          r->err = S->fatal;
-          release(S, r);
-        }
+        if (r) fail_request(S, r, kFailed, S->fatal.c_str());
       break;
     }
   }
@@ -451,6 +532,15 @@ int tsw_server_create(double dx, double dz, double dt, int n_sponge, double spon
   S->es.resize(max_active);
   TSW_TRY(S->d_tbshots.ensure(max_active * sizeof(TbShot)));
   TSW_TRY(S->d_eshots.ensure(max_active * sizeof(EShot)));
+  const size_t asz = Arena::up((size_t)budget_bytes);
+  if (hipMalloc(&S->arena_mem, asz) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("cannot reserve the %lld-byte HBM budget", budget_bytes);
+    return kErrMem;
+  }
+  S->arena.base = static_cast<char*>(S->arena_mem);
+  S->arena.size = asz;
+  S->arena.free_[0] = asz;
   Server* raw = S.release();
   raw->worker = std::thread(worker_loop, raw);
   *out = reinterpret_cast<tsw_server_t>(raw);
@@ -493,8 +583,9 @@ int tsw_server_submit(tsw_server_t h, const tsw_shot_desc* d, long long* id_out)
   r->rix.assign(d->rix, d->rix + d->nrec);
   std::lock_guard<std::mutex> lk(S->mu);
   TSW_ARG(!S->stop, "server is shutting down");
-  r->need = req_need(*r, S->k);
-  r->keep = 2 * 4 * (long long)n;
+  r->heavy_bytes = (size_t)req_heavy(*r, S->k);
+  r->keep = req_keep(*r);
+  r->need = (long long)r->heavy_bytes + r->keep;
   if (r->need > S->budget) {
     set_error("request needs %lld bytes, more than the %lld-byte budget", r->need, S->budget);
     return kErrMem;
@@ -557,10 +648,9 @@ int tsw_server_result(tsw_server_t h, long long id, float* image_out, float* ill
   }
   TSW_ARG(r.state == kDone, "request %lld is not done (state %d)", id, r.state);
   const size_t nb = (size_t)r.cells() * sizeof(float);
-  if (image_out) TSW_HIP(hipMemcpy(image_out, r.image.p, nb, hipMemcpyDeviceToHost));
-  if (illum_out) TSW_HIP(hipMemcpy(illum_out, r.illum.p, nb, hipMemcpyDeviceToHost));
-  r.image.release();
-  r.illum.release();
+  if (image_out) TSW_HIP(hipMemcpy(image_out, r.image, nb, hipMemcpyDeviceToHost));
+  if (illum_out) TSW_HIP(hipMemcpy(illum_out, r.illum, nb, hipMemcpyDeviceToHost));
+  release_keep(S, &r);
   S->mem_used -= r.keep;
   S->reqs.erase(it);
   return kOk;
@@ -620,6 +710,7 @@ int tsw_server_destroy(tsw_server_t h) {
   (void)hipStreamSynchronize(S->se);
   (void)hipStreamDestroy(S->sa);
   (void)hipStreamDestroy(S->se);
+  if (S->arena_mem) (void)hipFree(S->arena_mem);
   delete S;
   return kOk;
 }

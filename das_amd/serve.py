@@ -86,10 +86,12 @@ class RTMServer:
 
 
 def run_poisson(server: RTMServer, shots: Sequence, rate_hz: float, seed: int = 0,
-                keep_results: bool = False, log=None) -> dict:
+                keep_results: bool = False, log=None, make=None) -> dict:
     """Submits `shots` at seeded exponential inter-arrival times (mean
     1/rate_hz) on the host clock, waits for all, and reports latency
-    (arrival -> image ready, server clock) percentiles and throughput."""
+    (arrival -> image ready, server clock) percentiles and throughput.
+    With `make`, `shots` are lightweight specs and make(spec) builds each
+    request just before its arrival (the server copies it on submission)."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate_hz, len(shots))
     arrivals = np.cumsum(gaps) - gaps[0]
@@ -99,7 +101,7 @@ def run_poisson(server: RTMServer, shots: Sequence, rate_hz: float, seed: int = 
         delay = ta - (time.perf_counter() - t0)
         if delay > 0:
             time.sleep(delay)
-        ids.append(server.submit(s))
+        ids.append(server.submit(make(s) if make else s))
         if log and (i + 1) % 50 == 0:
             log(f"served: submitted {i + 1}/{len(shots)}")
     lat, results, failed = [], {}, 0

@@ -257,3 +257,65 @@ class Config2:
 
     def cell_updates(self):
         return sum(2 * s.nt * s.cells for s in self.shots)
+
+
+@dataclass
+class RequestSpec:
+    """A served request before its arrays are built (kind, window, length, seed)."""
+    kind: str
+    nz: int
+    nx: int
+    nt: int
+    x0: int
+    six: int
+    seed: int
+
+    @property
+    def cells(self):
+        return self.nx * self.nz
+
+
+class Config4:
+    """Served RTM: a seeded stream of virtual-shot requests, nz in
+    {256, 384, 512} x nx in {512, 1024, 2048}, nt ~ U{2000..8000}, 80% acoustic
+    and 20% elastic, all cropped from one shared faulted model, each with its
+    own seeded correlogram gather (receivers every 2 cells on row 2)."""
+
+    def __init__(self, n_requests=2000, seed=4, dx=5.0, dt=4e-4, k=2, n_sponge=30,
+                 elastic_fraction=0.2, f0=2.5, bank_traces=2048):
+        self.dx, self.dt, self.k, self.n_sponge, self.f0 = dx, dt, k, n_sponge, f0
+        rng = np.random.default_rng(seed)
+        self.global_nx = 2048 + 1024
+        self.nz_max = 512
+        vp = layered_faulted_vp(self.nz_max, self.global_nx, dx, seed)
+        self.vel = smooth(vp, 100.0, dx)           # acoustic migration model
+        self.ela = elastic_from_vp(self.vel)       # elastic migration model
+        self.specs = []
+        for i in range(n_requests):
+            kind = "elastic" if rng.random() < elastic_fraction else "acoustic"
+            nz = int(rng.choice([256, 384, 512]))
+            nx = int(rng.choice([512, 1024, 2048]))
+            nt = int(rng.integers(2000, 8001))
+            x0 = int(rng.integers(0, self.global_nx - nx + 1))
+            six = int(rng.integers(nx // 4, 3 * nx // 4))
+            self.specs.append(RequestSpec(kind, nz, nx, nt, x0, six, int(rng.integers(1 << 30))))
+        # bank of whitened 1-bit correlogram traces, 8000 samples
+        brng = np.random.default_rng(seed + 1)
+        offs = (brng.random(bank_traces) - 0.5) * 2 * 2048 * dx
+        self.bank = correlogram_gather(brng, 8000, dt, offs, f0=f0).T.copy()  # (traces, nt)
+
+    def make(self, spec: RequestSpec):
+        rng = np.random.default_rng(spec.seed)
+        rix = (np.arange(spec.nx // 2) * 2).astype(np.int32)
+        pick = rng.choice(self.bank.shape[0], len(rix), replace=len(rix) > self.bank.shape[0])
+        data = np.ascontiguousarray(self.bank[pick, :spec.nt].T)
+        src, _ = ricker_wavelet(self.f0, self.dt, spec.nt * self.dt)
+        win = (slice(0, spec.nz), slice(spec.x0, spec.x0 + spec.nx))
+        if spec.kind == "elastic":
+            m = tuple(np.ascontiguousarray(a[win]) for a in self.ela)
+            return ElasticShot(nx=spec.nx, nz=spec.nz, nt=spec.nt, model_fwd=m, model_bwd=m,
+                               src=src, six=spec.six, siz=2, data=data, rix=rix, riz=2,
+                               x0=spec.x0)
+        v = np.ascontiguousarray(self.vel[win])
+        return Shot(nx=spec.nx, nz=spec.nz, nt=spec.nt, vel_fwd=v, vel_bwd=v, src=src,
+                    six=spec.six, siz=2, data=data, rix=rix, riz=2, x0=spec.x0)
