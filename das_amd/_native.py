@@ -63,6 +63,13 @@ SIGNATURES = {
     "tsw_server_cancel": (I, [P, LL]),
     "tsw_server_stats": (I, [P] + [ctypes.POINTER(LL)] * 7 + [ctypes.POINTER(D)]),
     "tsw_server_destroy": (I, [P]),
+    "tsw_fwi_create": (I, [I, I, D, D, D, I, D, I, I, I, P, I, ctypes.POINTER(P)]),
+    "tsw_fwi_forward": (I, [P, P, P, ctypes.POINTER(D)]),
+    "tsw_fwi_backward": (I, [P, P, P, P]),
+    "tsw_fwi_predicted": (I, [P, I, P]),
+    "tsw_fwi_set_residual": (I, [P, I, P]),
+    "tsw_fwi_probe": (I, [P, I, P]),
+    "tsw_fwi_destroy": (I, [P]),
 }
 
 RUN_BATCHED = 0
@@ -74,6 +81,12 @@ class ShotDesc(ctypes.Structure):
     _fields_ = [("nx", I), ("nz", I), ("nt", I), ("vel_fwd", P), ("vel_bwd", P), ("src", P),
                 ("six", I), ("siz", I), ("data", P), ("rix", P), ("nrec", I), ("riz", I),
                 ("x0", I), ("z0", I), ("kind", I), ("ela_fwd", P * 4), ("ela_bwd", P * 4)]
+
+
+class FwiShot(ctypes.Structure):
+    """tsw_fwi_shot of include/tswrtm.h."""
+    _fields_ = [("nt", I), ("src", P), ("six", I), ("siz", I), ("rix", P), ("nrec", I),
+                ("riz", I), ("d_obs", P)]
 
 
 SHOT_ACOUSTIC = 0

@@ -30,6 +30,18 @@ extern "C" {
 typedef struct tsw_grid_s* tsw_grid_t;
 typedef struct tsw_batch_s* tsw_batch_t;
 typedef struct tsw_server_s* tsw_server_t;
+typedef struct tsw_fwi_s* tsw_fwi_t;
+
+/* One shot of a differentiable acoustic modelling problem.  src (nt) and
+ * d_obs (nt, nrec) are host arrays in the problem's precision. */
+typedef struct tsw_fwi_shot {
+  int nt;
+  const void* src;
+  int six, siz;
+  const int* rix;
+  int nrec, riz;
+  const void* d_obs;
+} tsw_fwi_shot;
 
 /* One virtual shot of a batch, on its own (nz, nx) grid.  The shot's grid is
  * the window [z0, z0+nz) x [x0, x0+nx) of a larger stacking grid. */
@@ -228,6 +240,31 @@ int tsw_server_stats(tsw_server_t server, long long* completed, long long* faile
                      long long* mem_used, long long* cell_updates, double* busy_s);
 /* Cancels everything outstanding, stops the worker and frees the server. */
 int tsw_server_destroy(tsw_server_t server);
+
+/* Waveform misfit and its exact discrete adjoint (no reference counterpart;
+ * the reference's back-propagation, wave_solver.py:188-242, images with the
+ * forward operator and is not an adjoint).  J(v) = 1/2 sum |R P - d_obs|^2 over
+ * the shots, with P from the reference's forward loop (wave_solver.py:157-183,
+ * wave2d.c:19-82) in precision 4 (FP32) or 8 (FP64).  The forward levels are
+ * checkpointed every `ckpt_interval` steps (0: ceil(sqrt(nt)); -1: every level
+ * stored, no recompute) and recomputed during the adjoint sweep.  Device
+ * pointers (vel, grad: nz*nx in the working precision) and `stream` (a
+ * hipStream_t, e.g. torch's current stream) are the caller's. */
+int tsw_fwi_create(int nx, int nz, double dx, double dz, double dt, int n_sponge,
+                   double sponge_alpha, int free_surface_top, int precision, int nshot,
+                   const tsw_fwi_shot* shots, int ckpt_interval, tsw_fwi_t* out);
+/* Forward modelling of every shot in velocity d_vel; returns J. */
+int tsw_fwi_forward(tsw_fwi_t fwi, const void* d_vel, void* stream, double* loss_out);
+/* dJ/dv into d_grad (adjoint sweep with recompute); needs tsw_fwi_forward first. */
+int tsw_fwi_backward(tsw_fwi_t fwi, const void* d_vel, void* d_grad, void* stream);
+/* Predicted data of a shot from the last forward, (nt, nrec), to the host. */
+int tsw_fwi_predicted(tsw_fwi_t fwi, int shot, void* host_out);
+/* Overwrites a shot's residual (nt, nrec) before tsw_fwi_backward (adjoint tests). */
+int tsw_fwi_set_residual(tsw_fwi_t fwi, int shot, const void* host_res);
+/* Adjoint field at the shot's source cell for every iteration of the last
+ * backward sweep, (nt), to the host (adjoint tests). */
+int tsw_fwi_probe(tsw_fwi_t fwi, int shot, void* host_out);
+int tsw_fwi_destroy(tsw_fwi_t fwi);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,11 @@ def lib():
                 [_P] * 5 + [_I, _I, _D, _D, _D, _I, _P, _I, _I, _I, _P, _I, _I, _I, _P, _P, _P])
             getattr(L, f"ora_elastic_backward_{sfx}").argtypes = (
                 [_P] * 5 + [_I, _I, _D, _D, _D, _I, _P, _I, _P, _I, _I, _P, _L, _P, _I])
+        for sfx in ("f32", "f64"):
+            f = getattr(L, f"ora_misfit_gradient_{sfx}")
+            f.argtypes = ([_P, _P, _I, _I, _D, _D, _D, _I, _P, _I, _I, _I, _P, _I, _I]
+                          + [_P] * 5)
+            f.restype = _D
         _lib = L
     return _lib
 
@@ -318,3 +323,26 @@ def elastic_migrate_shot(g: Grid, model_fwd, model_bwd, src, six, siz, data, rix
     img = np.zeros((g.nz, g.nx), dtype=T)
     elastic_backward(g, model_bwd, data, rix, riz, snaps, img, k)
     return img, ill
+
+
+def misfit_gradient(g: Grid, vel, src, six, siz, rix, riz, d_obs, dtype=np.float64, res=None):
+    """Misfit J = 1/2 |R P - d_obs|^2 of forward() and dJ/dvel by the discrete
+    adjoint with every level kept (rtm_oracle.cpp, misfit_gradient).  `res`,
+    when given, replaces the residual (adjoint tests).  Returns
+    (J, grad (nz,nx), pred (nt,nrec), probe (nt) = dJ/dU^m at the source)."""
+    T = np.dtype(dtype)
+    src = _c(src, T)
+    nt = len(src)
+    rix = _c(rix, np.int32)
+    vel = _c(vel, T)
+    d_obs = _c(d_obs, T)
+    res = None if res is None else _c(res, T)
+    sp = g.sponge(T)
+    grad = np.zeros((g.nz, g.nx), T)
+    pred = np.zeros((nt, len(rix)), T)
+    probe = np.zeros(nt, T)
+    loss = getattr(lib(), f"ora_misfit_gradient_{_sfx(T)}")(
+        _ptr(vel), _ptr(sp), g.nx, g.nz, g.dx, g.dz, g.dt, int(g.free_surface), _ptr(src), nt,
+        int(six), int(siz), _ptr(rix), len(rix), int(riz), _ptr(d_obs), _ptr(res), _ptr(grad),
+        _ptr(pred), _ptr(probe))
+    return loss, grad, pred, probe

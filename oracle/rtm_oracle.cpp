@@ -500,3 +500,115 @@ void ora_laplacian_f32(const float* img, float* out, int nx, int nz, double dx, 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Waveform misfit J = 1/2 sum |R P - d_obs|^2 of the forward loop above and
+// its gradient by the discrete adjoint, written out level by level with every
+// forward level kept in memory (small grids only).  There is no reference
+// counterpart: the reference's back-propagation (wave_solver.py:188-242) is
+// not an adjoint.  With U^m the post-injection level at iteration m and
+// P^{m+1} the level the step writes (d_m = R P^{m+1}):
+//   Z^{m+1} = S.(2U^m - U^{m-1} + C.(L U^m)) on I = [2,nz-2) x [2,nx-2),
+//   Z^{m+1} = U^{m-2} on the other cells E (never written: three buffers
+//             back), P^{m+1} = Q Z^{m+1} (free surface: row 0 = 0, row 1 =
+//             -row 2), U^{m+1} = P^{m+1} + source.
+// A^m = dJ/dU^m is accumulated scatter-wise from Pbar = dJ/dP^{m+1} =
+// A^{m+1} + R^T r_m, descending in m.  probe[m] = A^m at the source cell
+// (dJ/ds_m = inj * probe[m]).
+namespace {
+template <typename T>
+double misfit_gradient(const T* vel, const T* sponge, int nx, int nz, T dx, T dz, double dt_host,
+                       int fs, const T* src, int nt, int six, int siz, const int* rix, int nrec,
+                       int riz, const T* d_obs, const T* res_in, T* grad_v, T* pred_out,
+                       T* probe_out) {
+  const size_t n = (size_t)nx * nz;
+  const T dt = T(dt_host);
+  const T inj = T(dt_host * dt_host);
+  std::vector<T> U((size_t)nt * n), pred((size_t)nt * nrec), res((size_t)nt * nrec);
+  {
+    std::vector<T> b0(n, T(0)), b1(n, T(0)), b2(n, T(0));
+    T* prv = b0.data();
+    T* cur = b1.data();
+    T* nxt = b2.data();
+    for (int it = 0; it < nt; ++it) {
+      cur[(size_t)siz * nx + six] += src[it] * inj;
+      std::memcpy(&U[(size_t)it * n], cur, n * sizeof(T));
+      acoustic_step<T>(cur, prv, nxt, vel, sponge, nx, nz, dx, dz, dt, fs);
+      for (int r = 0; r < nrec; ++r) pred[(size_t)it * nrec + r] = nxt[(size_t)riz * nx + rix[r]];
+      T* t = prv;
+      prv = cur;
+      cur = nxt;
+      nxt = t;
+    }
+  }
+  double loss = 0;
+  for (size_t i = 0; i < res.size(); ++i) {
+    res[i] = res_in ? res_in[i] : pred[i] - d_obs[i];
+    loss += 0.5 * (double)res[i] * (double)res[i];
+  }
+  const T rdx2 = T(1) / (dx * dx), rdz2 = T(1) / (dz * dz);
+  const T c0 = K4<T>::c0, c1 = lap_c1<T>(), c2 = lap_c2<T>();
+  const long long off[9] = {0, -1, 1, -2, 2, -(long long)nx, nx, -2LL * nx, 2LL * nx};
+  const T wgt[9] = {c0 * rdx2 + c0 * rdz2, c1 * rdx2, c1 * rdx2, c2 * rdx2, c2 * rdx2,
+                    c1 * rdz2, c1 * rdz2, c2 * rdz2, c2 * rdz2};
+  auto interior = [&](int iz, int ix) { return iz >= 2 && iz < nz - 2 && ix >= 2 && ix < nx - 2; };
+  // A[m + 2] = dJ/dU^m for m = -2 .. nt
+  std::vector<std::vector<T>> A(nt + 3, std::vector<T>(n, T(0)));
+  std::vector<T> gC(n, T(0)), Pbar(n), zeta(n);
+  for (int it = nt - 1; it >= 0; --it) {
+    Pbar = A[it + 3];
+    for (int r = 0; r < nrec; ++r) Pbar[(size_t)riz * nx + rix[r]] += res[(size_t)it * nrec + r];
+    zeta = Pbar;
+    if (fs)
+      for (int ix = 0; ix < nx; ++ix) {
+        zeta[2 * (size_t)nx + ix] -= Pbar[(size_t)nx + ix];
+        zeta[ix] = T(0);
+        zeta[(size_t)nx + ix] = T(0);
+      }
+    std::vector<T>& a0 = A[it + 2];
+    std::vector<T>& a1 = A[it + 1];
+    std::vector<T>& a2 = A[it];
+    const T* u = &U[(size_t)it * n];
+    for (int iz = 0; iz < nz; ++iz)
+      for (int ix = 0; ix < nx; ++ix) {
+        const size_t c = (size_t)iz * nx + ix;
+        if (interior(iz, ix)) {
+          const T w = sponge[c] * zeta[c];
+          a0[c] += T(2) * w;
+          a1[c] -= w;
+          const T cw = ((vel[c] * vel[c]) * (dt * dt)) * w;
+          T lu = T(0);
+          for (int k = 0; k < 9; ++k) {
+            a0[c + off[k]] += wgt[k] * cw;
+            lu += wgt[k] * u[c + off[k]];
+          }
+          gC[c] += w * lu;
+        } else if (!(fs && iz < 2)) {
+          a2[c] += zeta[c];
+        }
+      }
+    if (probe_out) probe_out[it] = a0[(size_t)siz * nx + six];
+  }
+  for (size_t c = 0; c < n; ++c) grad_v[c] = gC[c] * ((T(2) * vel[c]) * (dt * dt));
+  if (pred_out) std::memcpy(pred_out, pred.data(), pred.size() * sizeof(T));
+  return loss;
+}
+}  // namespace
+
+extern "C" {
+double ora_misfit_gradient_f32(const float* vel, const float* sponge, int nx, int nz, double dx,
+                               double dz, double dt, int fs, const float* src, int nt, int six,
+                               int siz, const int* rix, int nrec, int riz, const float* d_obs,
+                               const float* res_in, float* grad_v, float* pred, float* probe) {
+  return misfit_gradient<float>(vel, sponge, nx, nz, (float)dx, (float)dz, dt, fs, src, nt, six,
+                                siz, rix, nrec, riz, d_obs, res_in, grad_v, pred, probe);
+}
+double ora_misfit_gradient_f64(const double* vel, const double* sponge, int nx, int nz,
+                               double dx, double dz, double dt, int fs, const double* src, int nt,
+                               int six, int siz, const int* rix, int nrec, int riz,
+                               const double* d_obs, const double* res_in, double* grad_v,
+                               double* pred, double* probe) {
+  return misfit_gradient<double>(vel, sponge, nx, nz, dx, dz, dt, fs, src, nt, six, siz, rix,
+                                 nrec, riz, d_obs, res_in, grad_v, pred, probe);
+}
+}  // extern "C"

@@ -1,0 +1,246 @@
+"""Misfit gradient by the hand-written adjoint (das_amd.fwi).
+
+The forward model is pinned to the existing path (FP32 data bit-identical to
+forward_propagate_acoustic) and to the oracle (FP64).  The adjoint has no
+counterpart in the reference; it is pinned by what the mathematics fixes:
+the dot-product identity <F s, r> = <s, F^T r> in FP64, central finite
+differences of J along seeded smooth directions in FP64, linearity over shots,
+and the bitwise equality of the checkpointed gradient with the one that keeps
+every level."""
+import numpy as np
+import pytest
+import torch
+from scipy import ndimage
+
+pytestmark = pytest.mark.gpu
+
+DX, DT, NSP = 5.0, 4e-4, 10
+NZ, NX, NT = 48, 80, 200
+
+
+def _models():
+    """Smooth background and a true model with a 15% step at 60 m depth
+    (reflections return within the record)."""
+    rng = np.random.default_rng(7)
+    v0 = 2000.0 + 150.0 * ndimage.gaussian_filter(rng.standard_normal((NZ, NX)), 3.0)
+    v_true = v0 * np.where(np.arange(NZ)[:, None] >= 12, 1.15, 1.0)
+    return v_true.astype(np.float32), v0.astype(np.float32)
+
+
+def _shots(v_obs=None, n=1, dtype=np.float64, fs=True, seed=0):
+    """Shots on the surface with receivers on every other column (columns 0
+    and 1 are never-updated edge cells); d_obs modelled in v_obs (zero data
+    when v_obs is None)."""
+    from das_amd.fwi import MisfitShot
+    from das_amd.wave_solver import ricker_wavelet
+
+    src, _ = ricker_wavelet(30.0, DT, NT * DT)
+    rix = np.arange(0, NX, 2).astype(np.int32)
+    out = []
+    for i in range(n):
+        six = 20 + 37 * i
+        out.append(MisfitShot(src=src.astype(dtype), six=six, siz=2, rix=rix, riz=2,
+                              d_obs=np.zeros((NT, len(rix)), dtype)))
+    if v_obs is not None:
+        op = _op(out, dtype, fs=fs)
+        op.forward_loss(_dev(v_obs, dtype))
+        for i, s in enumerate(out):
+            s.d_obs = op.predicted(i)
+        op.close()
+    return out
+
+
+def _op(shots, dtype=np.float64, ckpt=0, fs=True, nt=None):
+    from das_amd.fwi import WaveformMisfit
+
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    return WaveformMisfit(NX, NZ, DX, DX, DT, shots, n_sponge=NSP, free_surface_top=fs,
+                          dtype=tdt, ckpt_interval=ckpt)
+
+
+def _dev(a, dtype=np.float64):
+    return torch.tensor(np.asarray(a, dtype), device="cuda")
+
+
+def test_fp32_prediction_equals_forward_propagate():
+    from das_amd.wave_solver import WaveSolver2D
+
+    v_true, v0 = _models()
+    shots = _shots(None, n=2, dtype=np.float32)
+    op = _op(shots, np.float32)
+    loss = op.forward_loss(_dev(v_true, np.float32))
+    solver = WaveSolver2D(NX, NZ, DX, DX, DT, n_sponge=NSP)
+    total = 0.0
+    for i, s in enumerate(shots):
+        rec, _ = solver.forward_propagate_acoustic(v_true, s.src, s.six, s.siz, s.rix, s.riz)
+        got = op.predicted(i)
+        np.testing.assert_array_equal(got, rec)
+        total += 0.5 * float(np.sum(rec.astype(np.float64) ** 2))
+    assert loss == pytest.approx(total, rel=1e-12)
+
+
+@pytest.mark.parametrize("fs", [True, False])
+def test_fp64_prediction_matches_oracle(fs):
+    from oracle import oracle as O
+
+    v_true, _ = _models()
+    shots = _shots(None, n=1, fs=fs)
+    op = _op(shots, fs=fs)
+    op.forward_loss(_dev(v_true))
+    g = O.Grid(nx=NX, nz=NZ, dx=DX, dz=DX, dt=DT, n_sponge=NSP,
+               sponge_alpha=float(np.float32(0.0015)), free_surface=fs)
+    s = shots[0]
+    rec, _ = O.forward(g, v_true.astype(np.float64), s.src, s.six, s.siz, s.rix, s.riz,
+                       dtype=np.float64)
+    got = op.predicted(0)
+    err = np.linalg.norm(got - rec) / np.linalg.norm(rec)
+    assert err < 1e-13, err
+
+
+@pytest.mark.parametrize("fs,six,siz", [(True, 30, 2), (True, 1, 2), (False, 40, 3),
+                                        (False, 0, 4), (True, 77, 45)])
+def test_dot_product_fp64(fs, six, siz):
+    """<F s, r> = <s, F^T r>: F maps the source time function to the data,
+    F^T r is read from the adjoint field at the source cell; the adjoint
+    field also equals the oracle's."""
+    from das_amd.fwi import MisfitShot
+    from oracle import oracle as O
+
+    _, v0 = _models()
+    rng = np.random.default_rng(11 + six)
+    s = rng.standard_normal(NT)
+    rix = np.arange(0, NX, 3).astype(np.int32)
+    riz = 2 if fs else 3
+    shot = MisfitShot(src=s, six=six, siz=siz, rix=rix, riz=riz,
+                      d_obs=np.zeros((NT, len(rix))))
+    op = _op([shot], fs=fs)
+    v = _dev(v0)
+    op.forward_loss(v)
+    fs_ = op.predicted(0)
+    r = rng.standard_normal(fs_.shape)
+    op.set_residual(0, r)
+    op.gradient(v)
+    lam = op.adjoint_at_source(0)
+    inj = DT * DT
+    lhs = float(np.sum(fs_ * r))
+    rhs = inj * float(np.sum(s * lam))
+    assert lhs != 0.0
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs), (lhs, rhs)
+    g = O.Grid(nx=NX, nz=NZ, dx=DX, dz=DX, dt=DT, n_sponge=NSP,
+               sponge_alpha=float(np.float32(0.0015)), free_surface=fs)
+    _, _, _, probe = O.misfit_gradient(g, v0.astype(np.float64), s, six, siz, rix, riz,
+                                       np.zeros_like(r), res=r)
+    assert np.linalg.norm(lam - probe) <= 1e-12 * np.linalg.norm(probe)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 1e-4)])
+def test_gradient_matches_oracle(dtype, tol):
+    from oracle import oracle as O
+
+    v_true, v0 = _models()
+    shots = _shots(v_true, n=2, dtype=dtype)
+    op = _op(shots, dtype)
+    v = _dev(v0, dtype)
+    loss = op.forward_loss(v)
+    got = op.gradient(v).cpu().numpy()
+    g = O.Grid(nx=NX, nz=NZ, dx=DX, dz=DX, dt=DT, n_sponge=NSP,
+               sponge_alpha=float(np.float32(0.0015)), free_surface=True)
+    want, J = np.zeros((NZ, NX), dtype), 0.0
+    for s in shots:
+        j, gr, _, _ = O.misfit_gradient(g, v0.astype(dtype), s.src, s.six, s.siz, s.rix, s.riz,
+                                        s.d_obs, dtype=dtype)
+        want += gr
+        J += j
+    assert loss == pytest.approx(J, rel=max(tol, 1e-12))
+    got, want = got.astype(np.float64), want.astype(np.float64)  # no FP32 underflow in norms
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    print("gradient vs oracle (%s): rel L2 %.3e" % (np.dtype(dtype).name, err))
+    assert err < tol, err
+
+
+def _directions(n, seed=0, sigma_m=20.0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        phi = ndimage.gaussian_filter(rng.standard_normal((NZ, NX)), sigma_m / DX)
+        out.append(phi / np.abs(phi).max())
+    return out
+
+
+def test_gradient_matches_central_fd_fp64():
+    """Central differences of J along 20 seeded smooth perturbations
+    dv = eps * v * phi (max|phi| = 1) against <g, dv>, eps = 1e-3, 1e-4, 1e-5
+    relative: the error of an exact gradient falls as eps^2."""
+    v_true, v0 = _models()
+    shots = _shots(v_true, n=2)
+    op = _op(shots)
+    v = _dev(v0)
+    op.forward_loss(v)
+    g = op.gradient(v).cpu().numpy()
+    v064 = v0.astype(np.float64)
+    table = []
+    for phi in _directions(20):
+        row = []
+        for eps in (1e-3, 1e-4, 1e-5):
+            dv = eps * v064 * phi
+            fd = (op.forward_loss(_dev(v064 + dv)) - op.forward_loss(_dev(v064 - dv))) / 2.0
+            row.append(abs(fd - float(np.sum(g * dv))) / abs(fd))
+        table.append(row)
+    t = np.array(table)
+    print("FD relative error, eps = 1e-3 / 1e-4 / 1e-5: median %s, max %s"
+          % (np.median(t, 0), t.max(0)))
+    assert t[:, 2].max() < 1e-4, t
+    assert np.all((t[:, 2] < 1e-8) | (t[:, 1] / t[:, 2] > 30)), t
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_checkpointed_gradient_equals_stored(dtype):
+    v_true, v0 = _models()
+    shots = _shots(v_true, n=2, dtype=dtype)
+    grads = []
+    for ck in (-1, 0, 7, 1, NT + 5):
+        op = _op(shots, dtype, ckpt=ck)
+        grads.append(op.gradient(_dev(v0, dtype)).cpu().numpy())
+        op.close()
+    assert np.abs(grads[0]).max() > 0
+    for g in grads[1:]:
+        np.testing.assert_array_equal(g, grads[0])
+
+
+def test_shots_add_up_fp64():
+    v_true, v0 = _models()
+    shots = _shots(v_true, n=2)
+    v = _dev(v0)
+    both = _op(shots).gradient(v).cpu().numpy()
+    single = sum(_op([s]).gradient(v).cpu().numpy() for s in shots)
+    err = np.linalg.norm(both - single) / np.linalg.norm(single)
+    assert err < 1e-13, err
+
+
+def test_fp32_gradient_close_to_fp64():
+    v_true, v0 = _models()
+    g64 = _op(_shots(v_true, n=2)).gradient(_dev(v0)).cpu().numpy()
+    g32 = _op(_shots(v_true, n=2, dtype=np.float32), np.float32).gradient(
+        _dev(v0, np.float32)).cpu().numpy()
+    err = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
+    print("FP32 gradient vs FP64: rel L2 %.3e" % err)
+    assert err < 1e-3, err
+
+
+def test_autograd_function():
+    v_true, v0 = _models()
+    op = _op(_shots(v_true, n=1))
+    v = _dev(v0).requires_grad_(True)
+    loss = op(v)
+    (3.0 * loss).backward()
+    g = op.gradient(v.detach())
+    torch.testing.assert_close(v.grad, 3.0 * g, rtol=0, atol=0)
+    # a forward at another velocity between forward and backward: the
+    # backward recomputes at the saved velocity
+    w = _dev(v0 * 1.01).requires_grad_(True)
+    lw = op(w)
+    op.forward_loss(_dev(v0))
+    lw.backward()
+    torch.testing.assert_close(w.grad, op.gradient(w.detach()), rtol=0, atol=0)
+    with pytest.raises(ValueError):
+        op(_dev(v0, np.float32))
