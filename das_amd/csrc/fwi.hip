@@ -46,23 +46,30 @@ template <typename T> __host__ __device__ inline T fwi_c2() { return T(-1) / T(1
 template <> __host__ __device__ inline float fwi_c1<float>() { return 4.0f / 3.0f; }
 template <> __host__ __device__ inline float fwi_c2<float>() { return -1.0f / 12.0f; }
 
+// Per-shot constants of a batched launch (blockIdx.z = shot within a group).
+struct FwiMeta {
+  const void* src;   // (nt) source time function
+  const void* res;   // (nt, nrec) residual
+  void* pred;        // (nt, nrec) predicted data
+  void* probe;       // (nt) adjoint at the source cell
+  const int* roff;   // receiver CSR over columns (nx + 1)
+  const int* ridx;
+  int src_cell, riz, nrec, pad;
+};
+
 template <typename T>
 struct FwdArgs {
   int nx, nz, fs;
   T rdx2, rdz2, inj;
+  long long n;       // cells per shot: shot j's level is base + j * n
   const T* C;
   const T* S;
   const T* cur;
   const T* prv;
   T* nxt;
-  int src_cell, src_it;
-  const T* src;
-  int riz, nrec;
-  const int* roff;
-  const int* ridx;
-  T* rec;
-  int rec_it;
-  T* save;  // non-null: save = cur (u^n)
+  T* save;           // non-null: save + j * n = cur (u^n)
+  const FwiMeta* meta;
+  int src_it, rec_it;  // -1: none
 };
 
 template <typename T>
@@ -79,59 +86,69 @@ __device__ __forceinline__ T lap4(const T* p, int c, int nx, T rdx2, T rdz2) {
 }
 
 template <typename T>
-__device__ __forceinline__ void fwd_io(const FwdArgs<T>& s, int iz, int ix, T& v, bool& touched) {
-  if (iz == s.riz && s.roff) {
-    const int a = s.roff[ix], b = s.roff[ix + 1];
-    if (s.rec_it >= 0)
-      for (int r = a; r < b; ++r) s.rec[(long long)s.rec_it * s.nrec + s.ridx[r]] = v;
+__device__ __forceinline__ void fwd_io(const FwdArgs<T>& s, const FwiMeta& m, int iz, int ix,
+                                       T& v, bool& touched) {
+  if (s.rec_it >= 0 && iz == m.riz) {
+    const int a = m.roff[ix], b = m.roff[ix + 1];
+    T* rec = (T*)m.pred + (long long)s.rec_it * m.nrec;
+    for (int r = a; r < b; ++r) rec[m.ridx[r]] = v;
   }
-  if (s.src_it >= 0 && iz * s.nx + ix == s.src_cell) {
-    v = v + s.src[s.src_it] * s.inj;
+  if (s.src_it >= 0 && iz * s.nx + ix == m.src_cell) {
+    v = v + ((const T*)m.src)[s.src_it] * s.inj;
     touched = true;
   }
 }
 
-// One forward step (same operation order as the FP32 production kernel).
+// One forward step of every shot of a group (same operation order as the
+// FP32 production kernel, so FP32 data are bit-identical to it).
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwi_fwd(FwdArgs<T> s) {
   const int ix = blockIdx.x * 64 + threadIdx.x;
   const int iz = blockIdx.y * 4 + threadIdx.y;
   if (ix >= s.nx || iz >= s.nz) return;
+  const long long o = (long long)blockIdx.z * s.n;
+  const FwiMeta& m = s.meta[blockIdx.z];
+  const T* cur = s.cur + o;
+  T* nxt = s.nxt + o;
   const int nx = s.nx, c = iz * nx + ix;
-  if (s.save) s.save[c] = s.cur[c];
+  if (s.save) s.save[o + c] = cur[c];
   if (s.fs && iz < 2) return;
   const bool interior = iz >= 2 && iz < s.nz - 2 && ix >= 2 && ix < nx - 2;
   T v;
   if (interior) {
-    const T pc = s.cur[c];
-    v = (T(2) * pc - s.prv[c]) + s.C[c] * lap4(s.cur, c, nx, s.rdx2, s.rdz2);
+    const T pc = cur[c];
+    v = (T(2) * pc - s.prv[o + c]) + s.C[c] * lap4(cur, c, nx, s.rdx2, s.rdz2);
     v = v * s.S[c];
   } else {
-    v = s.nxt[c];
+    v = nxt[c];
   }
   bool t0 = false, t1 = false, t2 = false;
   if (s.fs && iz == 2) {
     T v0 = T(0);
-    fwd_io(s, 0, ix, v0, t0);
-    s.nxt[ix] = v0;
+    fwd_io(s, m, 0, ix, v0, t0);
+    nxt[ix] = v0;
     T v1 = -v;
-    fwd_io(s, 1, ix, v1, t1);
-    s.nxt[nx + ix] = v1;
+    fwd_io(s, m, 1, ix, v1, t1);
+    nxt[nx + ix] = v1;
   }
-  fwd_io(s, iz, ix, v, t2);
-  if (interior || t2) s.nxt[c] = v;
+  fwd_io(s, m, iz, ix, v, t2);
+  if (interior || t2) nxt[c] = v;
 }
 
-// iteration-0 injection: buf[cell] += src[0] * inj
+// iteration-0 injection: level[j][src_cell] += src_j[0] * inj, one thread per shot
 template <typename T>
-__global__ void k_fwi_inject0(T* buf, int cell, const T* src, T inj) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) buf[cell] = buf[cell] + src[0] * inj;
+__global__ void k_fwi_inject0(T* level, long long n, const FwiMeta* meta, int count, T inj) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  T* p = level + j * n + meta[j].src_cell;
+  *p = *p + ((const T*)meta[j].src)[0] * inj;
 }
 
 template <typename T>
 struct AdjArgs {
   int nx, nz, fs;
   T rdx2, rdz2;
+  long long n;
   const T* C;
   const T* S;
   const T* mu1;  // mu^{n+1}, complete
@@ -139,14 +156,10 @@ struct AdjArgs {
   T* mum1;       // mu^{n-1}
   T* mum2;       // mu^{n-2} (assigned)
   const T* un;   // forward u^n
-  T* grad;       // g_C
-  int riz, nrec;
-  const int* roff;
-  const int* ridx;
-  const T* resid;  // (nt, nrec) residual d - d_obs
-  int res_it;      // residual row added to mu^n (n-1), -1: none
-  T* probe;        // non-null: probe[probe_it] = dJ/du^n at probe_cell
-  int probe_cell, probe_it;
+  T* grad;       // g_C, one slice per shot of the group
+  const FwiMeta* meta;
+  int res_it;    // residual row added to mu^n (n-1), -1: none
+  int probe_it;
 };
 
 template <typename T>
@@ -156,21 +169,21 @@ __device__ __forceinline__ bool fwi_interior(int iz, int ix, int nz, int nx) {
 
 // zeta = Q^T mu^{n+1} at (iz, ix)
 template <typename T>
-__device__ __forceinline__ T fwi_zeta(const AdjArgs<T>& a, int iz, int ix) {
+__device__ __forceinline__ T fwi_zeta(const AdjArgs<T>& a, const T* mu1, int iz, int ix) {
   const int c = iz * a.nx + ix;
-  if (!a.fs) return a.mu1[c];
+  if (!a.fs) return mu1[c];
   if (iz < 2) return T(0);
-  if (iz == 2) return a.mu1[c] - a.mu1[c - a.nx];
-  return a.mu1[c];
+  if (iz == 2) return mu1[c] - mu1[c - a.nx];
+  return mu1[c];
 }
 
 // C.w at (iz, ix), zero outside the grid and outside I
 template <typename T>
-__device__ __forceinline__ T fwi_cw(const AdjArgs<T>& a, int iz, int ix) {
+__device__ __forceinline__ T fwi_cw(const AdjArgs<T>& a, const T* mu1, int iz, int ix) {
   if (iz < 0 || iz >= a.nz || ix < 0 || ix >= a.nx) return T(0);
   if (!fwi_interior<T>(iz, ix, a.nz, a.nx)) return T(0);
   const int c = iz * a.nx + ix;
-  return a.C[c] * (a.S[c] * fwi_zeta(a, iz, ix));
+  return a.C[c] * (a.S[c] * fwi_zeta(a, mu1, iz, ix));
 }
 
 template <typename T>
@@ -178,33 +191,36 @@ __global__ __launch_bounds__(256) void k_fwi_adj(AdjArgs<T> a) {
   const int ix = blockIdx.x * 64 + threadIdx.x;
   const int iz = blockIdx.y * 4 + threadIdx.y;
   if (ix >= a.nx || iz >= a.nz) return;
+  const long long o = (long long)blockIdx.z * a.n;
+  const FwiMeta& m = a.meta[blockIdx.z];
+  const T* mu1 = a.mu1 + o;
   const int nx = a.nx, c = iz * nx + ix;
   const T c0 = FwiK<T>::c0, c1 = fwi_c1<T>(), c2 = fwi_c2<T>();
   const bool in_i = fwi_interior<T>(iz, ix, a.nz, nx);
-  const T zeta = fwi_zeta(a, iz, ix);
+  const T zeta = fwi_zeta(a, mu1, iz, ix);
   const T w = in_i ? a.S[c] * zeta : T(0);
   // L^T (C.w) at c
-  T lx = c2 * (fwi_cw(a, iz, ix - 2) + fwi_cw(a, iz, ix + 2));
-  lx = lx + c1 * (fwi_cw(a, iz, ix - 1) + fwi_cw(a, iz, ix + 1));
-  const T cwc = fwi_cw(a, iz, ix);
+  T lx = c2 * (fwi_cw(a, mu1, iz, ix - 2) + fwi_cw(a, mu1, iz, ix + 2));
+  lx = lx + c1 * (fwi_cw(a, mu1, iz, ix - 1) + fwi_cw(a, mu1, iz, ix + 1));
+  const T cwc = fwi_cw(a, mu1, iz, ix);
   lx = (lx + c0 * cwc) * a.rdx2;
-  T lz = c2 * (fwi_cw(a, iz - 2, ix) + fwi_cw(a, iz + 2, ix));
-  lz = lz + c1 * (fwi_cw(a, iz - 1, ix) + fwi_cw(a, iz + 1, ix));
+  T lz = c2 * (fwi_cw(a, mu1, iz - 2, ix) + fwi_cw(a, mu1, iz + 2, ix));
+  lz = lz + c1 * (fwi_cw(a, mu1, iz - 1, ix) + fwi_cw(a, mu1, iz + 1, ix));
   lz = (lz + c0 * cwc) * a.rdz2;
-  T m0 = a.mu0[c] + (T(2) * w + (lx + lz));
+  T m0 = a.mu0[o + c] + (T(2) * w + (lx + lz));
   // probe: dJ/du^n at the source cell (complete here; the receiver term
   // below belongs to dJ/dP^n, not to the injected u^n)
-  if (a.probe && c == a.probe_cell) a.probe[a.probe_it] = m0;
-  if (a.res_it >= 0 && iz == a.riz) {
-    const int r0 = a.roff[ix], r1 = a.roff[ix + 1];
-    const T* row = a.resid + (long long)a.res_it * a.nrec;
-    for (int r = r0; r < r1; ++r) m0 = m0 + row[a.ridx[r]];
+  if (c == m.src_cell) ((T*)m.probe)[a.probe_it] = m0;
+  if (a.res_it >= 0 && iz == m.riz) {
+    const int r0 = m.roff[ix], r1 = m.roff[ix + 1];
+    const T* row = (const T*)m.res + (long long)a.res_it * m.nrec;
+    for (int r = r0; r < r1; ++r) m0 = m0 + row[m.ridx[r]];
   }
-  a.mu0[c] = m0;
-  a.mum1[c] = a.mum1[c] - w;
+  a.mu0[o + c] = m0;
+  a.mum1[o + c] = a.mum1[o + c] - w;
   const bool kept = !in_i && !(a.fs && iz < 2);
-  a.mum2[c] = kept ? zeta : T(0);
-  if (in_i) a.grad[c] = a.grad[c] + w * lap4(a.un, c, nx, a.rdx2, a.rdz2);
+  a.mum2[o + c] = kept ? zeta : T(0);
+  if (in_i) a.grad[o + c] = a.grad[o + c] + w * lap4(a.un + o, c, nx, a.rdx2, a.rdz2);
 }
 
 template <typename T>
@@ -214,17 +230,19 @@ __global__ void k_fwi_fill(T* p, long long n, T v) {
     p[i] = v;
 }
 
-// mu^{nt} = R^T r_{nt-1}
+// mu^{nt} = R^T r_{nt-1}, shot j = blockIdx.y
 template <typename T>
-__global__ void k_fwi_seed(T* mu, int nx, int riz, const int* roff, const int* ridx,
-                           const T* row) {
+__global__ void k_fwi_seed(T* mu, long long n, int nx, const FwiMeta* meta, int row_it) {
   const int ix = blockIdx.x * blockDim.x + threadIdx.x;
   if (ix >= nx) return;
-  const int r0 = roff[ix], r1 = roff[ix + 1];
+  const FwiMeta& m = meta[blockIdx.y];
+  const int r0 = m.roff[ix], r1 = m.roff[ix + 1];
   if (r0 == r1) return;
-  T m = mu[riz * nx + ix];
-  for (int r = r0; r < r1; ++r) m = m + row[ridx[r]];
-  mu[riz * nx + ix] = m;
+  const T* row = (const T*)m.res + (long long)row_it * m.nrec;
+  T* p = mu + blockIdx.y * n + (long long)m.riz * nx + ix;
+  T v = *p;
+  for (int r = r0; r < r1; ++r) v = v + row[m.ridx[r]];
+  *p = v;
 }
 
 template <typename T>
@@ -234,35 +252,75 @@ __global__ void k_fwi_coef(const T* v, T* C, long long n, T dt2) {
     C[i] = (v[i] * v[i]) * dt2;
 }
 
-// dJ/dv = g_C . 2 v dt^2, written to out (accumulated when acc != 0)
+// dJ/dv = (sum over the group slots of g_C) . 2 v dt^2, slots added in order
 template <typename T>
-__global__ void k_fwi_chain(const T* g, const T* v, T* out, long long n, T dt2, int acc) {
+__global__ void k_fwi_chain(const T* g, int slots, const T* v, T* out, long long n, T dt2) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
-    const T d = g[i] * ((T(2) * v[i]) * dt2);
-    out[i] = acc ? out[i] + d : d;
+    T acc = g[i];
+    for (int j = 1; j < slots; ++j) acc = acc + g[j * n + i];
+    out[i] = acc * ((T(2) * v[i]) * dt2);
   }
 }
 
+constexpr int kSumBlocks = 512;
+
+// res = pred - obs, and per-block partial sums of res^2 in double (fixed grid
+// and tree order: deterministic)
 template <typename T>
-__global__ void k_fwi_resid(const T* pred, const T* obs, T* res, long long n) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    res[i] = pred[i] - obs[i];
+__global__ __launch_bounds__(256) void k_fwi_resid(const T* pred, const T* obs, T* res,
+                                                   long long n, double* partial) {
+  __shared__ double red[256];
+  double acc = 0;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const T r = pred[i] - obs[i];
+    res[i] = r;
+    acc += (double)r * (double)r;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// loss[shot] = 1/2 sum of the partials, in order (one block)
+__global__ void k_fwi_loss(const double* partial, double* loss) {
+  __shared__ double red[256];
+  double acc = 0;
+  for (int i = threadIdx.x; i < kSumBlocks; i += 256) acc += partial[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *loss = 0.5 * red[0];
 }
 
 struct FwiShotHost {
   int nt, six, siz, riz, nrec;
-  DevBuf src, obs, pred, res, roff, ridx, ckpt, probe;
-  std::vector<double> host_res;
+  DevBuf src, obs, pred, res, roff, ridx, probe;
+};
+
+// Shots of equal nt advance together, one launch per step.
+struct FwiGroup {
+  int nt = 0;
+  std::vector<int> ids;
+  DevBuf meta;  // FwiMeta[count]
+  DevBuf ckpt;  // [nck][3][count][n]
+  int count() const { return (int)ids.size(); }
 };
 
 struct Fwi {
-  int nx = 0, nz = 0, fs = 1, prec = 4, kint = 1, n_sponge = 0;
+  int nx = 0, nz = 0, fs = 1, prec = 4, kint = 1, n_sponge = 0, cmax = 1;
   double dx = 0, dz = 0, dt = 0, alpha = 0;
   std::vector<FwiShotHost> shots;
-  DevBuf S, C, buf[3], mu[4], seg, gC, tmp;
-  double loss = 0;
+  std::vector<FwiGroup> groups;
+  DevBuf S, C, buf, mu, seg, gC, partial, loss;
+  double last_loss = 0;
   bool have_forward = false;
   size_t esz() const { return prec == 8 ? 8 : 4; }
   long long cells() const { return (long long)nx * nz; }
@@ -274,67 +332,96 @@ static unsigned g1(long long n) {
 }
 
 template <typename T>
+static FwdArgs<T> fwd_base(const Fwi* F, const FwiGroup& G) {
+  FwdArgs<T> s;
+  std::memset(&s, 0, sizeof s);
+  const T dxT = (T)F->dx, dzT = (T)F->dz;
+  s.nx = F->nx;
+  s.nz = F->nz;
+  s.fs = F->fs;
+  s.rdx2 = T(1) / (dxT * dxT);
+  s.rdz2 = T(1) / (dzT * dzT);
+  s.inj = (T)(F->dt * F->dt);
+  s.n = F->cells();
+  s.C = (const T*)F->C.p;
+  s.S = (const T*)F->S.p;
+  s.meta = (const FwiMeta*)G.meta.p;
+  return s;
+}
+
+// level b (0..2) of the group's working buffers
+template <typename T>
+static T* lvl(const Fwi* F, int b) {
+  return (T*)F->buf.p + (long long)b * F->cmax * F->cells();
+}
+template <typename T>
+static T* adj(const Fwi* F, int b) {
+  return (T*)F->mu.p + (long long)b * F->cmax * F->cells();
+}
+
+template <typename T>
+static void fwd_reset(const Fwi* F, const FwiGroup& G, hipStream_t st) {
+  const long long n = F->cells(), cn = (long long)G.count() * n;
+  for (int b = 0; b < 3; ++b)
+    hipLaunchKernelGGL(k_fwi_fill<T>, dim3(g1(cn)), dim3(256), 0, st, lvl<T>(F, b), cn, T(0));
+  hipLaunchKernelGGL(k_fwi_inject0<T>, dim3((G.count() + 63) / 64), dim3(64), 0, st,
+                     lvl<T>(F, 1), n, (const FwiMeta*)G.meta.p, G.count(), (T)(F->dt * F->dt));
+}
+
+template <typename T>
+static void fwd_step(const Fwi* F, const FwiGroup& G, FwdArgs<T>& s, int it, T* save, bool rec,
+                     hipStream_t st) {
+  s.prv = lvl<T>(F, it % 3);
+  s.cur = lvl<T>(F, (it + 1) % 3);
+  s.nxt = lvl<T>(F, (it + 2) % 3);
+  s.save = save;
+  s.src_it = it + 1 < G.nt ? it + 1 : -1;
+  s.rec_it = rec ? it : -1;
+  hipLaunchKernelGGL(k_fwi_fwd<T>, dim3((F->nx + 63) / 64, (F->nz + 3) / 4, G.count()),
+                     dim3(64, 4), 0, st, s);
+}
+
+template <typename T>
 static int fwi_forward_t(Fwi* F, const T* d_vel, hipStream_t st, bool store_all) {
   const long long n = F->cells();
   const T dtT = (T)F->dt;
-  const T dt2 = dtT * dtT;
-  hipLaunchKernelGGL(k_fwi_coef<T>, dim3(g1(n)), dim3(256), 0, st, d_vel, (T*)F->C.p, n, dt2);
-  const T dxT = (T)F->dx, dzT = (T)F->dz;
-  F->loss = 0;
-  for (auto& sh : F->shots) {
-    const int nt = sh.nt;
-    for (auto& b : F->buf)
-      hipLaunchKernelGGL(k_fwi_fill<T>, dim3(g1(n)), dim3(256), 0, st, (T*)b.p, n, T(0));
-    FwdArgs<T> s;
-    std::memset(&s, 0, sizeof s);
-    s.nx = F->nx;
-    s.nz = F->nz;
-    s.fs = F->fs;
-    s.rdx2 = T(1) / (dxT * dxT);
-    s.rdz2 = T(1) / (dzT * dzT);
-    s.inj = (T)(F->dt * F->dt);
-    s.C = (const T*)F->C.p;
-    s.S = (const T*)F->S.p;
-    s.src_cell = sh.siz * F->nx + sh.six;
-    s.src = (const T*)sh.src.p;
-    s.riz = sh.riz;
-    s.nrec = sh.nrec;
-    s.roff = sh.roff.i();
-    s.ridx = sh.ridx.i();
-    s.rec = (T*)sh.pred.p;
-    hipLaunchKernelGGL(k_fwi_inject0<T>, dim3(1), dim3(64), 0, st, (T*)F->buf[1].p, s.src_cell,
-                       (const T*)sh.src.p, s.inj);
-    const int K = F->kint;
-    for (int it = 0; it < nt; ++it) {
-      T* prv = (T*)F->buf[it % 3].p;
-      T* cur = (T*)F->buf[(it + 1) % 3].p;
-      T* nxt = (T*)F->buf[(it + 2) % 3].p;
-      s.save = nullptr;
+  hipLaunchKernelGGL(k_fwi_coef<T>, dim3(g1(n)), dim3(256), 0, st, d_vel, (T*)F->C.p, n,
+                     T(dtT * dtT));
+  const int K = F->kint;
+  for (auto& G : F->groups) {
+    const long long cn = (long long)G.count() * n;
+    fwd_reset<T>(F, G, st);
+    FwdArgs<T> s = fwd_base<T>(F, G);
+    for (int it = 0; it < G.nt; ++it) {
       if (!store_all && it % K == 0) {  // checkpoint (u^{n-2}, u^{n-1}, u^n)
-        T* ck = (T*)sh.ckpt.p + (long long)(it / K) * 3 * n;
-        TSW_HIP(hipMemcpyAsync(ck, nxt, n * sizeof(T), hipMemcpyDeviceToDevice, st));
-        TSW_HIP(hipMemcpyAsync(ck + n, prv, n * sizeof(T), hipMemcpyDeviceToDevice, st));
-        TSW_HIP(hipMemcpyAsync(ck + 2 * n, cur, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+        T* ck = (T*)G.ckpt.p + (long long)(it / K) * 3 * cn;
+        TSW_HIP(hipMemcpyAsync(ck, lvl<T>(F, (it + 2) % 3), cn * sizeof(T),
+                               hipMemcpyDeviceToDevice, st));
+        TSW_HIP(hipMemcpyAsync(ck + cn, lvl<T>(F, it % 3), cn * sizeof(T),
+                               hipMemcpyDeviceToDevice, st));
+        TSW_HIP(hipMemcpyAsync(ck + 2 * cn, lvl<T>(F, (it + 1) % 3), cn * sizeof(T),
+                               hipMemcpyDeviceToDevice, st));
       }
-      s.prv = prv;
-      s.cur = cur;
-      s.nxt = nxt;
-      s.src_it = it + 1 < nt ? it + 1 : -1;
-      s.rec_it = it;
-      hipLaunchKernelGGL(k_fwi_fwd<T>, dim3((F->nx + 63) / 64, (F->nz + 3) / 4), dim3(64, 4), 0,
-                         st, s);
+      fwd_step<T>(F, G, s, it, nullptr, true, st);
     }
-    const long long nr = (long long)nt * sh.nrec;
-    hipLaunchKernelGGL(k_fwi_resid<T>, dim3(g1(nr)), dim3(256), 0, st, (const T*)sh.pred.p,
-                       (const T*)sh.obs.p, (T*)sh.res.p, nr);
+    for (int j = 0; j < G.count(); ++j) {
+      const int id = G.ids[j];
+      const FwiShotHost& sh = F->shots[id];
+      const long long nr = (long long)sh.nt * sh.nrec;
+      hipLaunchKernelGGL(k_fwi_resid<T>, dim3(kSumBlocks), dim3(256), 0, st, (const T*)sh.pred.p,
+                         (const T*)sh.obs.p, (T*)sh.res.p, nr, (double*)F->partial.p);
+      hipLaunchKernelGGL(k_fwi_loss, dim3(1), dim3(256), 0, st, (const double*)F->partial.p,
+                         (double*)F->loss.p + id);
+    }
     TSW_HIP(hipGetLastError());
-    std::vector<T> h(nr);
-    TSW_HIP(hipMemcpyAsync(h.data(), sh.res.p, nr * sizeof(T), hipMemcpyDeviceToHost, st));
-    TSW_HIP(hipStreamSynchronize(st));
-    double acc = 0;
-    for (const T& x : h) acc += (double)x * (double)x;
-    F->loss += 0.5 * acc;
   }
+  std::vector<double> l(F->shots.size());
+  TSW_HIP(hipMemcpyAsync(l.data(), F->loss.p, l.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         st));
+  TSW_HIP(hipStreamSynchronize(st));
+  double total = 0;
+  for (double x : l) total += x;
+  F->last_loss = total;
   F->have_forward = true;
   return kOk;
 }
@@ -344,95 +431,64 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
   const long long n = F->cells();
   const T dtT = (T)F->dt;
   const T dt2 = dtT * dtT;
-  const T dxT = (T)F->dx, dzT = (T)F->dz;
-  hipLaunchKernelGGL(k_fwi_fill<T>, dim3(g1(n)), dim3(256), 0, st, (T*)F->gC.p, n, T(0));
-  for (auto& sh : F->shots) {
-    const int nt = sh.nt;
-    for (auto& m : F->mu)
-      hipLaunchKernelGGL(k_fwi_fill<T>, dim3(g1(n)), dim3(256), 0, st, (T*)m.p, n, T(0));
+  const long long gn = (long long)F->cmax * n;
+  hipLaunchKernelGGL(k_fwi_fill<T>, dim3(g1(gn)), dim3(256), 0, st, (T*)F->gC.p, gn, T(0));
+  for (auto& G : F->groups) {
+    const int nt = G.nt, cnt = G.count();
+    const long long cn = (long long)cnt * n;
+    for (int b = 0; b < 4; ++b)
+      hipLaunchKernelGGL(k_fwi_fill<T>, dim3(g1(cn)), dim3(256), 0, st, adj<T>(F, b), cn, T(0));
     // mu^{nt} lives in buffer nt % 4
-    hipLaunchKernelGGL(k_fwi_seed<T>, dim3((F->nx + 255) / 256), dim3(256), 0, st,
-                       (T*)F->mu[nt % 4].p, F->nx, sh.riz, sh.roff.i(), sh.ridx.i(),
-                       (const T*)sh.res.p + (long long)(nt - 1) * sh.nrec);
+    hipLaunchKernelGGL(k_fwi_seed<T>, dim3((F->nx + 255) / 256, cnt), dim3(256), 0, st,
+                       adj<T>(F, nt % 4), n, F->nx, (const FwiMeta*)G.meta.p, nt - 1);
     AdjArgs<T> a;
     std::memset(&a, 0, sizeof a);
+    const T dxT = (T)F->dx, dzT = (T)F->dz;
     a.nx = F->nx;
     a.nz = F->nz;
     a.fs = F->fs;
     a.rdx2 = T(1) / (dxT * dxT);
     a.rdz2 = T(1) / (dzT * dzT);
+    a.n = n;
     a.C = (const T*)F->C.p;
     a.S = (const T*)F->S.p;
     a.grad = (T*)F->gC.p;
-    a.riz = sh.riz;
-    a.nrec = sh.nrec;
-    a.roff = sh.roff.i();
-    a.ridx = sh.ridx.i();
-    a.resid = (const T*)sh.res.p;
-    a.probe = sh.probe.p ? (T*)sh.probe.p : nullptr;
-    a.probe_cell = sh.siz * F->nx + sh.six;
-    // forward recompute machinery
-    FwdArgs<T> s;
-    std::memset(&s, 0, sizeof s);
-    s.nx = F->nx;
-    s.nz = F->nz;
-    s.fs = F->fs;
-    s.rdx2 = a.rdx2;
-    s.rdz2 = a.rdz2;
-    s.inj = (T)(F->dt * F->dt);
-    s.C = a.C;
-    s.S = a.S;
-    s.src_cell = a.probe_cell;
-    s.src = (const T*)sh.src.p;
-    s.riz = -1;
-    s.rec_it = -1;
+    a.meta = (const FwiMeta*)G.meta.p;
+    FwdArgs<T> s = fwd_base<T>(F, G);
     const int K = store_all ? nt : F->kint;
     const int nseg = (nt + K - 1) / K;
     for (int sg = nseg - 1; sg >= 0; --sg) {
       const int n0 = sg * K, n1 = std::min(nt, n0 + K);
-      {
-        if (store_all) {
-          // one segment: the whole forward again from the zero initial state
-          for (auto& b : F->buf)
-            hipLaunchKernelGGL(k_fwi_fill<T>, dim3(g1(n)), dim3(256), 0, st, (T*)b.p, n, T(0));
-          hipLaunchKernelGGL(k_fwi_inject0<T>, dim3(1), dim3(64), 0, st, (T*)F->buf[1].p,
-                             s.src_cell, (const T*)sh.src.p, s.inj);
-        } else {
-          // restore (u^{n0-2}, u^{n0-1}, u^{n0}) into the buffers the step at n0 uses
-          const T* ck = (const T*)sh.ckpt.p + (long long)sg * 3 * n;
-          TSW_HIP(hipMemcpyAsync(F->buf[(n0 + 2) % 3].p, ck, n * sizeof(T),
-                                 hipMemcpyDeviceToDevice, st));
-          TSW_HIP(hipMemcpyAsync(F->buf[n0 % 3].p, ck + n, n * sizeof(T),
-                                 hipMemcpyDeviceToDevice, st));
-          TSW_HIP(hipMemcpyAsync(F->buf[(n0 + 1) % 3].p, ck + 2 * n, n * sizeof(T),
-                                 hipMemcpyDeviceToDevice, st));
-        }
-        for (int it = n0; it < n1; ++it) {
-          s.prv = (T*)F->buf[it % 3].p;
-          s.cur = (T*)F->buf[(it + 1) % 3].p;
-          s.nxt = (T*)F->buf[(it + 2) % 3].p;
-          s.src_it = it + 1 < nt ? it + 1 : -1;
-          s.save = (T*)F->seg.p + (long long)(it - n0) * n;
-          hipLaunchKernelGGL(k_fwi_fwd<T>, dim3((F->nx + 63) / 64, (F->nz + 3) / 4),
-                             dim3(64, 4), 0, st, s);
-        }
+      if (store_all) {
+        fwd_reset<T>(F, G, st);  // one segment: the whole forward again
+      } else {
+        // restore (u^{n0-2}, u^{n0-1}, u^{n0}) into the buffers the step at n0 uses
+        const T* ck = (const T*)G.ckpt.p + (long long)sg * 3 * cn;
+        TSW_HIP(hipMemcpyAsync(lvl<T>(F, (n0 + 2) % 3), ck, cn * sizeof(T),
+                               hipMemcpyDeviceToDevice, st));
+        TSW_HIP(hipMemcpyAsync(lvl<T>(F, n0 % 3), ck + cn, cn * sizeof(T),
+                               hipMemcpyDeviceToDevice, st));
+        TSW_HIP(hipMemcpyAsync(lvl<T>(F, (n0 + 1) % 3), ck + 2 * cn, cn * sizeof(T),
+                               hipMemcpyDeviceToDevice, st));
       }
+      for (int it = n0; it < n1; ++it)
+        fwd_step<T>(F, G, s, it, (T*)F->seg.p + (long long)(it - n0) * cn, false, st);
       for (int it = n1 - 1; it >= n0; --it) {
-        a.mu1 = (const T*)F->mu[(it + 1) % 4].p;
-        a.mu0 = (T*)F->mu[it % 4].p;
-        a.mum1 = (T*)F->mu[(it + 3) % 4].p;
-        a.mum2 = (T*)F->mu[(it + 2) % 4].p;
-        a.un = (const T*)F->seg.p + (long long)(it - n0) * n;
+        a.mu1 = adj<T>(F, (it + 1) % 4);
+        a.mu0 = adj<T>(F, it % 4);
+        a.mum1 = adj<T>(F, (it + 3) % 4);
+        a.mum2 = adj<T>(F, (it + 2) % 4);
+        a.un = (const T*)F->seg.p + (long long)(it - n0) * cn;
         a.res_it = it >= 1 ? it - 1 : -1;
         a.probe_it = it;
-        hipLaunchKernelGGL(k_fwi_adj<T>, dim3((F->nx + 63) / 64, (F->nz + 3) / 4), dim3(64, 4),
-                           0, st, a);
+        hipLaunchKernelGGL(k_fwi_adj<T>, dim3((F->nx + 63) / 64, (F->nz + 3) / 4, cnt),
+                           dim3(64, 4), 0, st, a);
       }
     }
     TSW_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_fwi_chain<T>, dim3(g1(n)), dim3(256), 0, st, (const T*)F->gC.p, d_vel,
-                     d_grad, n, dt2, 0);
+  hipLaunchKernelGGL(k_fwi_chain<T>, dim3(g1(n)), dim3(256), 0, st, (const T*)F->gC.p, F->cmax,
+                     d_vel, d_grad, n, dt2);
   TSW_HIP(hipGetLastError());
   TSW_HIP(hipStreamSynchronize(st));
   return kOk;
@@ -473,9 +529,13 @@ int tsw_fwi_create(int nx, int nz, double dx, double dz, double dt, int n_sponge
   int ntmax = 1;
   for (int i = 0; i < nshot; ++i) {
     const tsw_fwi_shot& d = shots[i];
-    if (!(d.nt >= 1 && d.src && d.six >= 0 && d.six < nx && d.siz >= 0 && d.siz < nz &&
-          d.riz >= 0 && d.riz < nz && d.nrec >= 1 && d.rix && d.d_obs))
-      return fail((set_error("shot %d: bad arguments", i), kErrArg));
+    if (!(d.nt >= 1 && d.src && d.d_obs && d.rix && d.nrec >= 1))
+      return fail((set_error("shot %d: need nt >= 1, nrec >= 1 and non-null arrays", i), kErrArg));
+    if (!(d.six >= 0 && d.six < nx && d.siz >= 0 && d.siz < nz))
+      return fail((set_error("shot %d: source (%d, %d) outside the %dx%d grid", i, d.siz, d.six,
+                             nz, nx), kErrArg));
+    if (!(d.riz >= 0 && d.riz < nz))
+      return fail((set_error("shot %d: receiver row %d outside the grid", i, d.riz), kErrArg));
     ntmax = std::max(ntmax, d.nt);
   }
   const bool store_all = ckpt_interval < 0;
@@ -498,12 +558,6 @@ int tsw_fwi_create(int nx, int nz, double dx, double dz, double dt, int n_sponge
     if (hipMemcpy(F->S.p, host.data(), n * es, hipMemcpyHostToDevice))
       return fail((set_error("sponge upload failed"), kErrHip));
   }
-  if ((rc = F->C.ensure(n * es)) || (rc = F->gC.ensure(n * es))) return fail(rc);
-  for (auto& b : F->buf)
-    if ((rc = b.ensure(n * es))) return fail(rc);
-  for (auto& m : F->mu)
-    if ((rc = m.ensure(n * es))) return fail(rc);
-  if ((rc = F->seg.ensure((size_t)(store_all ? ntmax : F->kint) * n * es))) return fail(rc);
   F->shots.resize(nshot);
   for (int i = 0; i < nshot; ++i) {
     const tsw_fwi_shot& d = shots[i];
@@ -520,16 +574,55 @@ int tsw_fwi_create(int nx, int nz, double dx, double dz, double dt, int n_sponge
         (rc = h.res.ensure(nr)) || (rc = h.roff.ensure(m.off.size() * sizeof(int))) ||
         (rc = h.ridx.ensure(m.idx.size() * sizeof(int))) || (rc = h.probe.ensure(d.nt * es)))
       return fail(rc);
-    if (!store_all) {
-      const int nck = (d.nt + F->kint - 1) / F->kint;
-      if ((rc = h.ckpt.ensure((size_t)nck * 3 * n * es))) return fail(rc);
-    }
     if (hipMemcpy(h.src.p, d.src, d.nt * es, hipMemcpyHostToDevice) ||
         hipMemcpy(h.obs.p, d.d_obs, nr, hipMemcpyHostToDevice) ||
         hipMemcpy(h.roff.p, m.off.data(), m.off.size() * sizeof(int), hipMemcpyHostToDevice) ||
         hipMemcpy(h.ridx.p, m.idx.data(), m.idx.size() * sizeof(int), hipMemcpyHostToDevice))
       return fail((set_error("shot %d upload failed", i), kErrHip));
+    // group by nt, in order of first appearance
+    FwiGroup* G = nullptr;
+    for (auto& g : F->groups)
+      if (g.nt == d.nt) G = &g;
+    if (!G) {
+      F->groups.emplace_back();
+      G = &F->groups.back();
+      G->nt = d.nt;
+    }
+    G->ids.push_back(i);
   }
+  F->cmax = 1;
+  for (auto& G : F->groups) {
+    F->cmax = std::max(F->cmax, G.count());
+    std::vector<FwiMeta> meta(G.count());
+    for (int j = 0; j < G.count(); ++j) {
+      const FwiShotHost& h = F->shots[G.ids[j]];
+      FwiMeta& m = meta[j];
+      std::memset(&m, 0, sizeof m);
+      m.src = h.src.p;
+      m.res = h.res.p;
+      m.pred = h.pred.p;
+      m.probe = h.probe.p;
+      m.roff = h.roff.i();
+      m.ridx = h.ridx.i();
+      m.src_cell = h.siz * nx + h.six;
+      m.riz = h.riz;
+      m.nrec = h.nrec;
+    }
+    if ((rc = G.meta.ensure(meta.size() * sizeof(FwiMeta)))) return fail(rc);
+    if (hipMemcpy(G.meta.p, meta.data(), meta.size() * sizeof(FwiMeta), hipMemcpyHostToDevice))
+      return fail((set_error("metadata upload failed"), kErrHip));
+    if (!store_all) {
+      const int nck = (G.nt + F->kint - 1) / F->kint;
+      if ((rc = G.ckpt.ensure((size_t)nck * 3 * G.count() * n * es))) return fail(rc);
+    }
+  }
+  const size_t cn = (size_t)F->cmax * n * es;
+  if ((rc = F->C.ensure(n * es)) || (rc = F->gC.ensure(cn)) || (rc = F->buf.ensure(3 * cn)) ||
+      (rc = F->mu.ensure(4 * cn)) ||
+      (rc = F->seg.ensure((size_t)(store_all ? ntmax : std::min(F->kint, ntmax)) * cn)) ||
+      (rc = F->partial.ensure(kSumBlocks * sizeof(double))) ||
+      (rc = F->loss.ensure(nshot * sizeof(double))))
+    return fail(rc);
   F->kint = store_all ? -1 : F->kint;
   *out = reinterpret_cast<tsw_fwi_t>(F);
   return kOk;
@@ -543,7 +636,7 @@ int tsw_fwi_forward(tsw_fwi_t h, const void* d_vel, void* stream, double* loss_o
   const bool store_all = F->kint < 0;
   const int rc = F->prec == 8 ? fwi_forward_t<double>(F, (const double*)d_vel, st, store_all)
                               : fwi_forward_t<float>(F, (const float*)d_vel, st, store_all);
-  if (rc == kOk && loss_out) *loss_out = F->loss;
+  if (rc == kOk && loss_out) *loss_out = F->last_loss;
   return rc;
 }
 
