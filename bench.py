@@ -138,6 +138,127 @@ def served_leg(args, acoustic_gcell, elastic_gcell, kernel, log):
             "kernel": kernel}
 
 
+FWI_BYTES_PER_SHOT_CELL_STEP = 88.0  # forward 20 B + recompute 24 B + adjoint 44 B (FP32)
+
+
+def fwi_leg(args, log):
+    """Misfit gradient (forward, checkpointed recompute and discrete adjoint)
+    of 16 shots on 300 x 800, nt 5000, FP32, timed on resident inputs.  With
+    --fwi-check, also the FP64 central-difference check along 20 seeded smooth
+    directions and the correlation of the gradient with the stacked RTM image
+    of the residuals."""
+    import torch
+
+    from das_amd.fwi import MisfitShot, WaveformMisfit
+    from das_amd.workloads import Config3
+
+    cfg = Config3()
+    src = cfg.source()
+    geo = cfg.geometry()
+
+    def make_op(dtype, d_obs=None):
+        npdt = np.float64 if dtype == torch.float64 else np.float32
+        shots = [MisfitShot(src=src, six=six, siz=siz, rix=rix, riz=riz,
+                            d_obs=(np.zeros((cfg.nt, len(rix)), npdt) if d_obs is None
+                                   else d_obs[i]))
+                 for i, (six, siz, rix, riz) in enumerate(geo)]
+        return WaveformMisfit(cfg.nx, cfg.nz, cfg.dx, cfg.dx, cfg.dt, shots,
+                              n_sponge=cfg.n_sponge, dtype=dtype)
+
+    def observed(dtype):
+        op = make_op(dtype)
+        op.forward_loss(torch.tensor(cfg.true_vel, dtype=dtype, device="cuda"))
+        d = [op.predicted(i) for i in range(len(geo))]
+        op.close()
+        return d
+
+    t0 = time.time()
+    d32 = observed(torch.float32)
+    op = make_op(torch.float32, d32)
+    v = torch.tensor(cfg.start_vel, dtype=torch.float32, device="cuda")
+    log(f"fwi: data modelled, operator resident in {time.time() - t0:.1f}s")
+    op.forward_loss(v)
+    op.gradient(v)
+    torch.cuda.synchronize()
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss = op.forward_loss(v)
+        g32 = op.gradient(v)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    steps = len(geo) * cfg.nt * cfg.cells()
+    ach = FWI_BYTES_PER_SHOT_CELL_STEP * steps / el / 1e9
+    out = {"workload": "misfit gradient, 16 shots, 300x800, nt=5000, dx=5 m, dt=0.5 ms, "
+                       "400 receivers, FP32, sqrt(nt)=71-step checkpoints + recompute",
+           "gradient_s": round(el, 4), "shots_per_s": round(len(geo) / el, 2),
+           "gcell_updates_per_s": round(3 * steps / el / 1e9, 2),
+           "effective_gbs": round(ach, 1), "frac_of_hbm_peak": round(ach / HBM_PEAK_GBS, 4),
+           "algorithmic_bytes_per_shot_cell_step": FWI_BYTES_PER_SHOT_CELL_STEP,
+           "loss": loss}
+    log(f"fwi: gradient in {el:.3f}s")
+    if args.fwi_check:
+        out["check"] = fwi_check(cfg, geo, src, make_op, observed, op, g32, v, d32, log)
+    op.close()
+    return out
+
+
+def fwi_check(cfg, geo, src, make_op, observed, op32, g32, v32, d32, log):
+    import torch
+    from scipy import ndimage
+
+    from das_amd.rtm_engine import TSWRTMEngine
+    from das_amd.velocity_flood import FloodModels
+    from das_amd.wave_solver import WaveSolver2D
+
+    t0 = time.time()
+    d64 = observed(torch.float64)
+    op = make_op(torch.float64, d64)
+    v0 = cfg.start_vel.astype(np.float64)
+    v = torch.tensor(v0, device="cuda")
+    op.forward_loss(v)
+    g = op.gradient(v).cpu().numpy()
+    rng = np.random.default_rng(2024)
+    errs = []
+    for i in range(20):
+        phi = ndimage.gaussian_filter(rng.standard_normal((cfg.nz, cfg.nx)), 100.0 / cfg.dx)
+        phi /= np.abs(phi).max()
+        row = []
+        for eps in (1e-3, 1e-4):
+            dv = eps * v0 * phi
+            jp = op.forward_loss(torch.tensor(v0 + dv, device="cuda"))
+            jm = op.forward_loss(torch.tensor(v0 - dv, device="cuda"))
+            fd = (jp - jm) / 2.0
+            row.append(abs(fd - float(np.sum(g * dv))) / abs(fd))
+        errs.append(row)
+        log(f"fwi check: direction {i}: rel err {row[0]:.2e} (eps 1e-3), {row[1]:.2e} (1e-4)")
+    e = np.array(errs)
+    g32n = g32.cpu().numpy().astype(np.float64)
+    fp32_vs_fp64 = float(np.linalg.norm(g32n - g) / np.linalg.norm(g))
+    op.close()
+    # stacked RTM image of the FP32 residuals, migrated in the start model
+    solver = WaveSolver2D(cfg.nx, cfg.nz, cfg.dx, cfg.dx, cfg.dt, n_sponge=cfg.n_sponge)
+    eng = TSWRTMEngine(solver)
+    fm = FloodModels(cfg.start_vel, cfg.start_vel, cfg.start_vel, 0, 0)
+    img = np.zeros((cfg.nz, cfg.nx))
+    op32.forward_loss(v32)
+    for i, (six, siz, rix, riz) in enumerate(geo):
+        res = op32.predicted(i) - d32[i]
+        im, _ = eng.migrate_shot(fm, src, six, siz, res, rix, riz, 2)
+        img += im
+    m = solver.sponge > 0.99
+    m[:3, :] = False
+    corr = float(np.corrcoef(g[m], img[m])[0, 1])
+    log(f"fwi check done in {time.time() - t0:.1f}s")
+    return {"fd_directions": 20, "fd_smoothing_m": 100.0,
+            "fd_rel_err_eps1e-3": {"median": float(np.median(e[:, 0])),
+                                   "max": float(e[:, 0].max())},
+            "fd_rel_err_eps1e-4": {"median": float(np.median(e[:, 1])),
+                                   "max": float(e[:, 1].max())},
+            "fp32_vs_fp64_gradient_rel_l2": fp32_vs_fp64,
+            "corr_gradient_vs_rtm_image_of_residual": corr}
+
+
 def traffic_from_profiles():
     """HBM bytes per launch of the step kernel from the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
@@ -163,6 +284,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-elastic", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-fwi", action="store_true")
+    ap.add_argument("--fwi-check", action="store_true",
+                    help="FP64 finite-difference check of the gradient (about a minute)")
     ap.add_argument("--served-requests", type=int, default=400)
     ap.add_argument("--served-load", type=float, default=0.8)
     ap.add_argument("--served-slots", type=int, default=32)
@@ -252,6 +376,8 @@ def main():
     batch.close()
     if not args.no_elastic:
         out["elastic"] = elastic_leg(args, log)
+    if not args.no_fwi:
+        out["fwi"] = fwi_leg(args, log)
     if not args.no_served:
         el = out.get("elastic", {}).get("value", 50.0)
         out["served"] = served_leg(args, value, el, args.kernel, log)

@@ -260,6 +260,40 @@ class Config2:
 
 
 @dataclass
+class Config3:
+    """Misfit gradient: 300 x 800 at 5 m, dt 0.5 ms, nt 5000, 16 surface shots
+    (Ricker 2.5 Hz) over 400 receivers every 10 m; observed data modelled in
+    the two-layer faulted model, gradient taken at its 100 m smoothing."""
+    nz: int = 300
+    nx: int = 800
+    dx: float = 5.0
+    dt: float = 5e-4
+    nt: int = 5000
+    n_shots: int = 16
+    n_sponge: int = 30
+    f0: float = 2.5
+    seed: int = 0
+    true_vel: np.ndarray = field(default=None, repr=False)
+    start_vel: np.ndarray = field(default=None, repr=False)
+
+    def __post_init__(self):
+        self.true_vel = faulted_two_layer(self.nz, self.nx, self.dx, self.seed)
+        self.start_vel = smooth(self.true_vel, 100.0, self.dx)
+
+    def source(self):
+        return ricker_wavelet(self.f0, self.dt, self.nt * self.dt)[0]
+
+    def geometry(self):
+        """[(six, siz, rix, riz)] of the shots."""
+        rix = (np.arange(self.nx // 2) * 2).astype(np.int32)
+        sx = np.linspace(self.n_sponge + 10, self.nx - self.n_sponge - 11, self.n_shots)
+        return [(int(round(x)), 2, rix, 2) for x in sx]
+
+    def cells(self):
+        return self.nx * self.nz
+
+
+@dataclass
 class RequestSpec:
     """A served request before its arrays are built (kind, window, length, seed)."""
     kind: str
