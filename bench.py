@@ -290,7 +290,8 @@ def main():
     ap.add_argument("--served-requests", type=int, default=400)
     ap.add_argument("--served-load", type=float, default=0.8)
     ap.add_argument("--served-slots", type=int, default=32)
-    ap.add_argument("--cpu-budget", type=float, default=2.5, help="G cell-updates")
+    ap.add_argument("--cpu-budget", type=float, default=25.0,
+                    help="G cell-updates of the CPU baseline sample (about 10 s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU design "
