@@ -13,14 +13,15 @@
 // C = (v*v)*dt^2, S the sponge and L the 4th-order Laplacian.  The misfit is
 // J = 1/2 sum_n |d_n - d_obs_n|^2.
 //
-// Adjoint (exact transpose, backward in time): with mu^m = dJ/du^m,
-//   zeta = Q^T mu^{n+1},  w = S.zeta on I (0 elsewhere),
-//   mu^n     += 2 w + L^T (C.w) + R^T r_{n-1}
-//   mu^{n-1} -= w
-//   mu^{n-2}  = zeta on E (first contribution, assigned)
-//   g_C      += w . (L u^n) on I,      dJ/dv = g_C . 2 v dt^2.
-// L is symmetric (central stencil), so L^T is the same stencil applied to the
-// zero-extended field.  Four adjoint buffers rotate with period 4.
+// Adjoint (exact transpose, backward in time), in gather form: with
+// A^m = dJ/du^m, Lam^m = dJ/dP^m = A^m + R^T r_{m-1} and
+// w^m = S.Q^T Lam^m on I (0 elsewhere),
+//   A^n = 2 w^{n+1} + L^T (C.w^{n+1}) - w^{n+2} + [Q^T Lam^{n+3} on E]
+//   g_C += w^{n+1} . (L u^n) on I,      dJ/dv = g_C . 2 v dt^2,
+// so each step reads Lam^{n+1} (stencil), Lam^{n+2} and, on the border cells
+// only, Lam^{n+3}, and writes Lam^n: four buffers rotate with period 4.  L is
+// symmetric (central stencil), so L^T is the same stencil applied to the
+// zero-extended field C.w, which each block stages in LDS with its halo.
 //
 // The forward levels u^n are not stored: every ceil(sqrt(nt)) steps the three
 // levels (u^{n-2}, u^{n-1}, u^n) are checkpointed, and the backward sweep
@@ -85,6 +86,25 @@ __device__ __forceinline__ T lap4(const T* p, int c, int nx, T rdx2, T rdz2) {
   return lx + lz;
 }
 
+// Blocks cover 64 columns x kTZ rows (256 threads, 4 rows each); the field
+// read through the stencil is staged in LDS with a 2-cell halo.
+constexpr int kTZ = 16;
+constexpr int kLW = 64 + 4;
+constexpr int kLH = kTZ + 4;
+
+template <typename T>
+__device__ __forceinline__ T lap4_lds(const T* t, int l, T rdx2, T rdz2) {
+  const T c0 = FwiK<T>::c0, c1 = fwi_c1<T>(), c2 = fwi_c2<T>();
+  const T pc = t[l];
+  T lx = c2 * (t[l - 2] + t[l + 2]);
+  lx = lx + c1 * (t[l - 1] + t[l + 1]);
+  lx = (lx + c0 * pc) * rdx2;
+  T lz = c2 * (t[l - 2 * kLW] + t[l + 2 * kLW]);
+  lz = lz + c1 * (t[l - kLW] + t[l + kLW]);
+  lz = (lz + c0 * pc) * rdz2;
+  return lx + lz;
+}
+
 template <typename T>
 __device__ __forceinline__ void fwd_io(const FwdArgs<T>& s, const FwiMeta& m, int iz, int ix,
                                        T& v, bool& touched) {
@@ -103,36 +123,48 @@ __device__ __forceinline__ void fwd_io(const FwdArgs<T>& s, const FwiMeta& m, in
 // FP32 production kernel, so FP32 data are bit-identical to it).
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwi_fwd(FwdArgs<T> s) {
-  const int ix = blockIdx.x * 64 + threadIdx.x;
-  const int iz = blockIdx.y * 4 + threadIdx.y;
-  if (ix >= s.nx || iz >= s.nz) return;
+  __shared__ T tile[kLH * kLW];
+  const int nx = s.nx, nz = s.nz;
+  const int x0 = blockIdx.x * 64, z0 = blockIdx.y * kTZ;
   const long long o = (long long)blockIdx.z * s.n;
-  const FwiMeta& m = s.meta[blockIdx.z];
   const T* cur = s.cur + o;
+  for (int i = threadIdx.y * 64 + threadIdx.x; i < kLH * kLW; i += 256) {
+    const int gz = z0 - 2 + i / kLW, gx = x0 - 2 + i % kLW;
+    tile[i] = (gz >= 0 && gz < nz && gx >= 0 && gx < nx) ? cur[gz * nx + gx] : T(0);
+  }
+  __syncthreads();
+  const int ix = x0 + threadIdx.x;
+  if (ix >= nx) return;
+  const FwiMeta& m = s.meta[blockIdx.z];
   T* nxt = s.nxt + o;
-  const int nx = s.nx, c = iz * nx + ix;
-  if (s.save) s.save[o + c] = cur[c];
-  if (s.fs && iz < 2) return;
-  const bool interior = iz >= 2 && iz < s.nz - 2 && ix >= 2 && ix < nx - 2;
-  T v;
-  if (interior) {
-    const T pc = cur[c];
-    v = (T(2) * pc - s.prv[o + c]) + s.C[c] * lap4(cur, c, nx, s.rdx2, s.rdz2);
-    v = v * s.S[c];
-  } else {
-    v = nxt[c];
+  for (int k = 0; k < kTZ / 4; ++k) {
+    const int lz = threadIdx.y + 4 * k, iz = z0 + lz;
+    if (iz >= nz) break;
+    const int c = iz * nx + ix;
+    const int l = (lz + 2) * kLW + threadIdx.x + 2;
+    const T pc = tile[l];
+    if (s.save) s.save[o + c] = pc;
+    if (s.fs && iz < 2) continue;
+    const bool interior = iz >= 2 && iz < nz - 2 && ix >= 2 && ix < nx - 2;
+    T v;
+    if (interior) {
+      v = (T(2) * pc - s.prv[o + c]) + s.C[c] * lap4_lds(tile, l, s.rdx2, s.rdz2);
+      v = v * s.S[c];
+    } else {
+      v = nxt[c];
+    }
+    bool t0 = false, t1 = false, t2 = false;
+    if (s.fs && iz == 2) {
+      T v0 = T(0);
+      fwd_io(s, m, 0, ix, v0, t0);
+      nxt[ix] = v0;
+      T v1 = -v;
+      fwd_io(s, m, 1, ix, v1, t1);
+      nxt[nx + ix] = v1;
+    }
+    fwd_io(s, m, iz, ix, v, t2);
+    if (interior || t2) nxt[c] = v;
   }
-  bool t0 = false, t1 = false, t2 = false;
-  if (s.fs && iz == 2) {
-    T v0 = T(0);
-    fwd_io(s, m, 0, ix, v0, t0);
-    nxt[ix] = v0;
-    T v1 = -v;
-    fwd_io(s, m, 1, ix, v1, t1);
-    nxt[nx + ix] = v1;
-  }
-  fwd_io(s, m, iz, ix, v, t2);
-  if (interior || t2) nxt[c] = v;
 }
 
 // iteration-0 injection: level[j][src_cell] += src_j[0] * inj, one thread per shot
@@ -151,14 +183,14 @@ struct AdjArgs {
   long long n;
   const T* C;
   const T* S;
-  const T* mu1;  // mu^{n+1}, complete
-  T* mu0;        // mu^n
-  T* mum1;       // mu^{n-1}
-  T* mum2;       // mu^{n-2} (assigned)
-  const T* un;   // forward u^n
-  T* grad;       // g_C, one slice per shot of the group
+  const T* lam1;  // Lam^{n+1}
+  const T* lam2;  // Lam^{n+2}
+  const T* lam3;  // Lam^{n+3}
+  T* lam0;        // Lam^n (written)
+  const T* un;    // forward u^n
+  T* grad;        // g_C, one slice per shot of the group
   const FwiMeta* meta;
-  int res_it;    // residual row added to mu^n (n-1), -1: none
+  int res_it;     // residual row added to Lam^n (n-1), -1: none
   int probe_it;
 };
 
@@ -167,60 +199,70 @@ __device__ __forceinline__ bool fwi_interior(int iz, int ix, int nz, int nx) {
   return iz >= 2 && iz < nz - 2 && ix >= 2 && ix < nx - 2;
 }
 
-// zeta = Q^T mu^{n+1} at (iz, ix)
+// (Q^T lam) at cell c = (iz, ix): the free surface zeroes rows 0/1 and
+// mirrors row 1 into row 2
 template <typename T>
-__device__ __forceinline__ T fwi_zeta(const AdjArgs<T>& a, const T* mu1, int iz, int ix) {
-  const int c = iz * a.nx + ix;
-  if (!a.fs) return mu1[c];
+__device__ __forceinline__ T fwi_zeta(int fs, const T* lam, int c, int iz, int nx) {
+  if (!fs) return lam[c];
   if (iz < 2) return T(0);
-  if (iz == 2) return mu1[c] - mu1[c - a.nx];
-  return mu1[c];
-}
-
-// C.w at (iz, ix), zero outside the grid and outside I
-template <typename T>
-__device__ __forceinline__ T fwi_cw(const AdjArgs<T>& a, const T* mu1, int iz, int ix) {
-  if (iz < 0 || iz >= a.nz || ix < 0 || ix >= a.nx) return T(0);
-  if (!fwi_interior<T>(iz, ix, a.nz, a.nx)) return T(0);
-  const int c = iz * a.nx + ix;
-  return a.C[c] * (a.S[c] * fwi_zeta(a, mu1, iz, ix));
+  if (iz == 2) return lam[c] - lam[c - nx];
+  return lam[c];
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwi_adj(AdjArgs<T> a) {
-  const int ix = blockIdx.x * 64 + threadIdx.x;
-  const int iz = blockIdx.y * 4 + threadIdx.y;
-  if (ix >= a.nx || iz >= a.nz) return;
+  __shared__ T cw[kLH * kLW];  // C.w^{n+1}, zero outside I
+  const int nx = a.nx, nz = a.nz;
+  const int x0 = blockIdx.x * 64, z0 = blockIdx.y * kTZ;
   const long long o = (long long)blockIdx.z * a.n;
-  const FwiMeta& m = a.meta[blockIdx.z];
-  const T* mu1 = a.mu1 + o;
-  const int nx = a.nx, c = iz * nx + ix;
-  const T c0 = FwiK<T>::c0, c1 = fwi_c1<T>(), c2 = fwi_c2<T>();
-  const bool in_i = fwi_interior<T>(iz, ix, a.nz, nx);
-  const T zeta = fwi_zeta(a, mu1, iz, ix);
-  const T w = in_i ? a.S[c] * zeta : T(0);
-  // L^T (C.w) at c
-  T lx = c2 * (fwi_cw(a, mu1, iz, ix - 2) + fwi_cw(a, mu1, iz, ix + 2));
-  lx = lx + c1 * (fwi_cw(a, mu1, iz, ix - 1) + fwi_cw(a, mu1, iz, ix + 1));
-  const T cwc = fwi_cw(a, mu1, iz, ix);
-  lx = (lx + c0 * cwc) * a.rdx2;
-  T lz = c2 * (fwi_cw(a, mu1, iz - 2, ix) + fwi_cw(a, mu1, iz + 2, ix));
-  lz = lz + c1 * (fwi_cw(a, mu1, iz - 1, ix) + fwi_cw(a, mu1, iz + 1, ix));
-  lz = (lz + c0 * cwc) * a.rdz2;
-  T m0 = a.mu0[o + c] + (T(2) * w + (lx + lz));
-  // probe: dJ/du^n at the source cell (complete here; the receiver term
-  // below belongs to dJ/dP^n, not to the injected u^n)
-  if (c == m.src_cell) ((T*)m.probe)[a.probe_it] = m0;
-  if (a.res_it >= 0 && iz == m.riz) {
-    const int r0 = m.roff[ix], r1 = m.roff[ix + 1];
-    const T* row = (const T*)m.res + (long long)a.res_it * m.nrec;
-    for (int r = r0; r < r1; ++r) m0 = m0 + row[m.ridx[r]];
+  const T* lam1 = a.lam1 + o;
+  for (int i = threadIdx.y * 64 + threadIdx.x; i < kLH * kLW; i += 256) {
+    const int gz = z0 - 2 + i / kLW, gx = x0 - 2 + i % kLW;
+    T v = T(0);
+    if (fwi_interior<T>(gz, gx, nz, nx)) {
+      const int c = gz * nx + gx;
+      v = a.C[c] * (a.S[c] * fwi_zeta(a.fs, lam1, c, gz, nx));
+    }
+    cw[i] = v;
   }
-  a.mu0[o + c] = m0;
-  a.mum1[o + c] = a.mum1[o + c] - w;
-  const bool kept = !in_i && !(a.fs && iz < 2);
-  a.mum2[o + c] = kept ? zeta : T(0);
-  if (in_i) a.grad[o + c] = a.grad[o + c] + w * lap4(a.un + o, c, nx, a.rdx2, a.rdz2);
+  __syncthreads();
+  const int ix = x0 + threadIdx.x;
+  if (ix >= nx) return;
+  const FwiMeta& m = a.meta[blockIdx.z];
+  const T c0 = FwiK<T>::c0, c1 = fwi_c1<T>(), c2 = fwi_c2<T>();
+  for (int k = 0; k < kTZ / 4; ++k) {
+    const int lz = threadIdx.y + 4 * k, iz = z0 + lz;
+    if (iz >= nz) break;
+    const int c = iz * nx + ix;
+    const int l = (lz + 2) * kLW + threadIdx.x + 2;
+    // L^T (C.w) at c
+    T lx = c2 * (cw[l - 2] + cw[l + 2]);
+    lx = lx + c1 * (cw[l - 1] + cw[l + 1]);
+    lx = (lx + c0 * cw[l]) * a.rdx2;
+    T lz2 = c2 * (cw[l - 2 * kLW] + cw[l + 2 * kLW]);
+    lz2 = lz2 + c1 * (cw[l - kLW] + cw[l + kLW]);
+    lz2 = (lz2 + c0 * cw[l]) * a.rdz2;
+    T acc;
+    if (fwi_interior<T>(iz, ix, nz, nx)) {
+      const T sc = a.S[c];
+      const T w1 = sc * fwi_zeta(a.fs, lam1, c, iz, nx);
+      const T w2 = sc * fwi_zeta(a.fs, a.lam2 + o, c, iz, nx);
+      acc = (T(2) * w1 + (lx + lz2)) - w2;
+      a.grad[o + c] = a.grad[o + c] + w1 * lap4(a.un + o, c, nx, a.rdx2, a.rdz2);
+    } else {
+      acc = lx + lz2;
+      if (!(a.fs && iz < 2)) acc = acc + fwi_zeta(a.fs, a.lam3 + o, c, iz, nx);
+    }
+    // probe: A^n = dJ/du^n at the source cell (before the receiver term,
+    // which belongs to dJ/dP^n)
+    if (c == m.src_cell) ((T*)m.probe)[a.probe_it] = acc;
+    if (a.res_it >= 0 && iz == m.riz) {
+      const int r0 = m.roff[ix], r1 = m.roff[ix + 1];
+      const T* row = (const T*)m.res + (long long)a.res_it * m.nrec;
+      for (int r = r0; r < r1; ++r) acc = acc + row[m.ridx[r]];
+    }
+    a.lam0[o + c] = acc;
+  }
 }
 
 template <typename T>
@@ -377,7 +419,7 @@ static void fwd_step(const Fwi* F, const FwiGroup& G, FwdArgs<T>& s, int it, T* 
   s.save = save;
   s.src_it = it + 1 < G.nt ? it + 1 : -1;
   s.rec_it = rec ? it : -1;
-  hipLaunchKernelGGL(k_fwi_fwd<T>, dim3((F->nx + 63) / 64, (F->nz + 3) / 4, G.count()),
+  hipLaunchKernelGGL(k_fwi_fwd<T>, dim3((F->nx + 63) / 64, (F->nz + kTZ - 1) / kTZ, G.count()),
                      dim3(64, 4), 0, st, s);
 }
 
@@ -438,7 +480,7 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
     const long long cn = (long long)cnt * n;
     for (int b = 0; b < 4; ++b)
       hipLaunchKernelGGL(k_fwi_fill<T>, dim3(g1(cn)), dim3(256), 0, st, adj<T>(F, b), cn, T(0));
-    // mu^{nt} lives in buffer nt % 4
+    // Lam^{nt} = R^T r_{nt-1} lives in buffer nt % 4; Lam^{nt+1..nt+3} = 0
     hipLaunchKernelGGL(k_fwi_seed<T>, dim3((F->nx + 255) / 256, cnt), dim3(256), 0, st,
                        adj<T>(F, nt % 4), n, F->nx, (const FwiMeta*)G.meta.p, nt - 1);
     AdjArgs<T> a;
@@ -474,14 +516,14 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
       for (int it = n0; it < n1; ++it)
         fwd_step<T>(F, G, s, it, (T*)F->seg.p + (long long)(it - n0) * cn, false, st);
       for (int it = n1 - 1; it >= n0; --it) {
-        a.mu1 = adj<T>(F, (it + 1) % 4);
-        a.mu0 = adj<T>(F, it % 4);
-        a.mum1 = adj<T>(F, (it + 3) % 4);
-        a.mum2 = adj<T>(F, (it + 2) % 4);
+        a.lam0 = adj<T>(F, it % 4);
+        a.lam1 = adj<T>(F, (it + 1) % 4);
+        a.lam2 = adj<T>(F, (it + 2) % 4);
+        a.lam3 = adj<T>(F, (it + 3) % 4);
         a.un = (const T*)F->seg.p + (long long)(it - n0) * cn;
         a.res_it = it >= 1 ? it - 1 : -1;
         a.probe_it = it;
-        hipLaunchKernelGGL(k_fwi_adj<T>, dim3((F->nx + 63) / 64, (F->nz + 3) / 4, cnt),
+        hipLaunchKernelGGL(k_fwi_adj<T>, dim3((F->nx + 63) / 64, (F->nz + kTZ - 1) / kTZ, cnt),
                            dim3(64, 4), 0, st, a);
       }
     }
