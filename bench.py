@@ -138,7 +138,7 @@ def served_leg(args, acoustic_gcell, elastic_gcell, kernel, log):
             "kernel": kernel}
 
 
-FWI_BYTES_PER_SHOT_CELL_STEP = 88.0  # forward 20 B + recompute 24 B + adjoint 44 B (FP32)
+FWI_BYTES_PER_SHOT_CELL_STEP = 76.0  # forward 20 B + recompute 24 B + adjoint 32 B (FP32)
 
 
 def fwi_leg(args, log):
