@@ -471,6 +471,10 @@ int tsw_migrate_shot(tsw_grid_t h, const float* vel_fwd, const float* vel_bwd, c
   TSW_ARG(g && vel_fwd && vel_bwd && src && image_out && illum_out, "null argument");
   TSW_TRY(check_shot_args(g, nt, nrec, riz, k));
   TSW_ARG(nt_src >= 1, "source length must be >= 1");
+  // The reference runs the forward over len(source_ts) and the back-propagation
+  // over receiver_data.shape[0] (rtm_engine.py:76-104, wave_solver.py:146,201);
+  // unequal lengths silently pair shifted snapshots, so they are rejected.
+  TSW_ARG(nt_src == nt, "source length %d differs from receiver data length %d", nt_src, nt);
   TSW_ARG(six >= 0 && six < g->nx && siz >= 0 && siz < g->nz, "source (%d, %d) outside grid",
           six, siz);
   TSW_ARG(nrec == 0 || (rix && data), "receivers need rix and data");
@@ -513,6 +517,8 @@ int tsw_migrate_shots(tsw_grid_t h, const float* vel_fwd, const float* vel_bwd, 
   for (int i = 0; i < nshot; ++i) {
     TSW_TRY(check_shot_args(g, nts[i], nrec, riz, k));
     TSW_ARG(nt_srcs[i] >= 1 && srcs[i], "shot %d: empty source", i);
+    TSW_ARG(nt_srcs[i] == nts[i], "shot %d: source length %d differs from data length %d", i,
+            nt_srcs[i], nts[i]);
     TSW_ARG(sixs[i] >= 0 && sixs[i] < g->nx && sizs[i] >= 0 && sizs[i] < g->nz,
             "shot %d: source (%d, %d) outside grid", i, sixs[i], sizs[i]);
     TSW_ARG(nrec == 0 || datas[i], "shot %d: data missing", i);

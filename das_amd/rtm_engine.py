@@ -66,6 +66,12 @@ class TSWRTMEngine:
             raise ValueError("snapshot_interval must be >= 1")
         if len(src) == 0 or data.shape[0] == 0:
             raise ValueError("empty source or receiver data")
+        if len(src) != data.shape[0]:
+            # the reference would run the forward over len(source_ts) and the
+            # back-propagation over receiver_data.shape[0], pairing shifted
+            # snapshots without notice (rtm_engine.py:76-104)
+            raise ValueError(f"source_ts has {len(src)} samples but receiver_data has "
+                             f"{data.shape[0]}; they must be equal")
         return src, six, siz, data, rix, riz, k
 
     def migrate_shot(self, flood_models: FloodModels, source_ts: np.ndarray, source_ix: int,
@@ -101,8 +107,6 @@ class TSWRTMEngine:
         src, six, siz, data, rix, riz, k = self._shot_args(
             source_ts, source_ix, source_iz, receiver_data, receiver_ix, receiver_iz,
             snapshot_interval)
-        if len(src) != data.shape[0]:
-            raise ValueError("elastic migration needs source and data of the same length")
         mf = s._elastic_model(model_fwd)
         mb = s._elastic_model(model_bwd)
         PF = (N.P * 4)(*[N.ptr(a) for a in mf])

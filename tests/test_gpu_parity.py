@@ -213,6 +213,14 @@ def test_errors_and_injected_failure(engine):
     assert np.isfinite(rec).all()
     with pytest.raises(ValueError):
         TSWRTMEngine(s).migrate_shot(flood_models_from(vel), src, 5, 2, rec, np.array([1]), 2)
+    # unequal source and data lengths would pair shifted snapshots: rejected
+    with pytest.raises(ValueError, match="must be equal"):
+        TSWRTMEngine(s).migrate_shot(flood_models_from(vel), src[:9], 5, 2, rec, np.array([1, 2]), 2)
+    img, ill = np.zeros((16, 20), np.float32), np.zeros((16, 20), np.float32)
+    rix = np.array([1, 2], np.int32)
+    rc = engine.tsw_migrate_shot(s._grid, N.ptr(vel), N.ptr(vel), N.ptr(src), 9, 5, 2, N.ptr(rec),
+                                 10, N.ptr(rix), 2, 2, 2, N.ptr(img), N.ptr(ill))
+    assert rc == -1 and "differs from receiver data length" in N.last_error()
     h = N.P()
     assert engine.tsw_grid_create(3, 16, 5.0, 5.0, 5e-4, 3, 0.0015, 1, ctypes.byref(h)) == -1
     assert "nx and nz" in N.last_error()
