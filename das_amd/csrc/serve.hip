@@ -23,6 +23,7 @@
 #include <thread>
 #include <vector>
 
+#include "arena.h"
 #include "elabatch.h"
 #include "post.h"
 #include "tb.h"
@@ -31,51 +32,6 @@
 namespace tsw {
 
 enum ReqState { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3, kCancelled = 4 };
-
-// First-fit sub-allocator over one device allocation made at server start:
-// per-request hipMalloc / hipFree of multi-GB snapshot arenas would stall the
-// serving loop for milliseconds each.  Free blocks are coalesced on release.
-struct Arena {
-  char* base = nullptr;
-  size_t size = 0;
-  std::map<size_t, size_t> free_;  // offset -> bytes
-  static size_t up(size_t b) { return (b + 255) & ~(size_t)255; }
-  bool fits(size_t bytes) const {
-    bytes = up(bytes);
-    for (const auto& f : free_)
-      if (f.second >= bytes) return true;
-    return false;
-  }
-  char* alloc(size_t bytes) {
-    bytes = up(bytes);
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second < bytes) continue;
-      const size_t off = it->first, rest = it->second - bytes;
-      free_.erase(it);
-      if (rest) free_[off + bytes] = rest;
-      return base + off;
-    }
-    return nullptr;
-  }
-  void release(char* p, size_t bytes) {
-    if (!p) return;
-    bytes = up(bytes);
-    size_t off = (size_t)(p - base);
-    auto nx = free_.lower_bound(off);
-    if (nx != free_.end() && off + bytes == nx->first) {  // merge with the next block
-      bytes += nx->second;
-      nx = free_.erase(nx);
-    }
-    if (nx != free_.begin()) {  // merge with the previous block
-      auto pv = std::prev(nx);
-      if (pv->first + pv->second == off) {
-        pv->second += bytes;
-        return;
-      }
-    }
-    free_[off] = bytes;
-  }
-};
 
 struct Req {
   long long id = 0;
@@ -416,7 +372,7 @@ static void worker_loop(Server* S) {
         }
       if (s < 0) break;
       char* h = S->arena.alloc(r->heavy_bytes);
-      char* kb = h ? S->arena.alloc((size_t)r->keep) : nullptr;
+      char* kb = h ? S->arena.alloc_high((size_t)r->keep) : nullptr;
       if (!kb) {
         S->arena.release(h, r->heavy_bytes);
         break;
