@@ -259,6 +259,65 @@ def fwi_check(cfg, geo, src, make_op, observed, op32, g32, v32, d32, log):
             "corr_gradient_vs_rtm_image_of_residual": corr}
 
 
+FP64_PEAK_TFLOPS = 78.6
+
+
+def gather_leg(args, log):
+    """Virtual-shot gather construction before migration: correlation of 64
+    virtual-source channels with 800 receiver channels over 10 min of 1-bit
+    50 Hz data (30000 samples), lags up to 3.5 s (nlag 175), timed on resident
+    data; then derivative, causal branch and resampling to 0.5 ms of one
+    source's gather (host round trip, reported beside)."""
+    import ctypes
+
+    import torch
+
+    from das_amd import _native as N
+    from das_amd.interferometry import build_virtual_shot_gather, resample_gather_to_sim_dt
+
+    nt, nch, nsrc, nrec, dt, max_lag = 30000, 1600, 64, 800, 0.02, 3.5
+    nlag = int(round(max_lag / dt))
+    g = torch.Generator(device="cpu").manual_seed(7)
+    data = torch.sign(torch.randn(nt, nch, generator=g)).to(torch.float32).cuda()
+    src = (np.linspace(20, nch // 4, nsrc)).astype(np.int32)
+    rec = np.arange(nch // 3, nch // 3 + nrec).astype(np.int32)
+    out = torch.empty(nsrc, 2 * nlag + 1, nrec, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def run():
+        N.check(N.lib().tsw_xcorr_device(ctypes.c_void_p(data.data_ptr()), nt, nch, N.ptr(src),
+                                         nsrc, N.ptr(rec), nrec, nlag, 0,
+                                         ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream)))
+
+    run()
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    macs = nsrc * nrec * (2 * nlag + 1) * nt
+    tflops = 2.0 * macs / el / 1e12
+    ccf = out[0].cpu().numpy()
+    lags = np.arange(-nlag, nlag + 1) * dt
+    t0 = time.perf_counter()
+    vs = build_virtual_shot_gather(ccf, lags, 0, int(src[0]), np.arange(nch) * 4.0, rec)
+    res = resample_gather_to_sim_dt(vs.shot_gather, vs.dt, 5e-4, 7000)
+    post = time.perf_counter() - t0
+    log(f"gathers: correlation {el * 1e3:.1f} ms ({tflops:.1f} TFLOP/s FP64)")
+    return {"workload": f"virtual-shot gathers: {nsrc} source x {nrec} receiver channels, "
+                        f"{nt} samples at 50 Hz (1-bit), lags -3.5..3.5 s ({2 * nlag + 1})",
+            "correlation_ms": round(el * 1e3, 3), "pairs_per_s": round(nsrc * nrec / el, 1),
+            "roofline": {"bound": "fp64", "achieved": round(tflops, 2),
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
+                         "kernel": "k_xcorr (FP64 lag sums of exact FP32 products)",
+                         "flops_per_launch": 2 * macs},
+            "derivative_branch_resample_ms": round(post * 1e3, 3),
+            "resampled_shape": list(res.shape)}
+
+
 def traffic_from_profiles():
     """HBM bytes per launch of the step kernel from the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
@@ -285,6 +344,7 @@ def main():
     ap.add_argument("--no-elastic", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-fwi", action="store_true")
+    ap.add_argument("--no-gathers", action="store_true")
     ap.add_argument("--fwi-check", action="store_true",
                     help="FP64 finite-difference check of the gradient (about a minute)")
     ap.add_argument("--served-requests", type=int, default=400)
@@ -379,6 +439,8 @@ def main():
         out["elastic"] = elastic_leg(args, log)
     if not args.no_fwi:
         out["fwi"] = fwi_leg(args, log)
+    if not args.no_gathers:
+        out["gathers"] = gather_leg(args, log)
     if not args.no_served:
         el = out.get("elastic", {}).get("value", 50.0)
         out["served"] = served_leg(args, value, el, args.kernel, log)

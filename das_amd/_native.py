@@ -70,6 +70,11 @@ SIGNATURES = {
     "tsw_fwi_set_residual": (I, [P, I, P]),
     "tsw_fwi_probe": (I, [P, I, P]),
     "tsw_fwi_destroy": (I, [P]),
+    "tsw_xcorr": (I, [P, I, I, P, I, P, I, I, I, P]),
+    "tsw_xcorr_device": (I, [P, I, I, P, I, P, I, I, I, P, P]),
+    "tsw_time_derivative": (I, [P, I, I, D, I, P]),
+    "tsw_green_branch": (I, [P, I, I, I, I, P, ctypes.POINTER(I)]),
+    "tsw_resample_linear": (I, [P, I, I, D, D, I, P]),
 }
 
 RUN_BATCHED = 0

@@ -266,6 +266,36 @@ int tsw_fwi_set_residual(tsw_fwi_t fwi, int shot, const void* host_res);
 int tsw_fwi_probe(tsw_fwi_t fwi, int shot, void* host_out);
 int tsw_fwi_destroy(tsw_fwi_t fwi);
 
+/* Virtual-shot gather construction (the input of back-injection).
+ *
+ * Cross-correlations of source channels with receiver channels of one block
+ * of continuous data; replaces compute_cross_correlation
+ * (tsw_rtm/interferometry.py:26-79) for several sources at once and the
+ * stacking over files (run_das_pipeline.py:202-213, accumulate != 0 adds in
+ * FP32).  data: (nt, nch) float32 row-major; out: (nsrc, 2*nlag+1, nrec), lag
+ * -nlag..nlag, out[s][j][r] = sum_t data[t][src_ch[s]] * data[t+j-nlag][rec_ch[r]]
+ * (exact products, FP64 sums, one rounding to FP32). */
+int tsw_xcorr(const float* data, int nt, int nch, const int* src_ch, int nsrc, const int* rec_ch,
+              int nrec, int nlag, int accumulate, float* out);
+/* Same on device pointers (d_data, d_out) and a caller stream; channel lists on the host. */
+int tsw_xcorr_device(const float* d_data, int nt, int nch, const int* src_ch, int nsrc,
+                     const int* rec_ch, int nrec, int nlag, int accumulate, float* d_out,
+                     void* stream);
+/* time_derivative_greens (interferometry.py:82-104) of (nl, ncol) float32:
+ * scheme 0 "central" (np.gradient, edge order 1), 1 spectral (irfft of
+ * i*omega*rfft over the next power of two). */
+int tsw_time_derivative(const float* in, int nl, int ncol, double dt, int scheme, float* out);
+/* Branch of build_virtual_shot_gather (interferometry.py:140-155): 0 causal,
+ * 1 anticausal (reversed), 2 symmetric (average); *nout must equal the
+ * branch's row count. */
+int tsw_green_branch(const float* in, int nl, int ncol, int zero_idx, int branch, float* out,
+                     int* nout);
+/* resample_gather_to_sim_dt (run_das_pipeline.py:48-55): linear
+ * interpolation of (nt_orig, ncol) from step dt_orig to target_nt samples of
+ * step dt_sim, 0 outside the original time range. */
+int tsw_resample_linear(const float* in, int nt_orig, int ncol, double dt_orig, double dt_sim,
+                        int target_nt, float* out);
+
 #ifdef __cplusplus
 }
 #endif
