@@ -121,12 +121,15 @@ def served_leg(args, acoustic_gcell, elastic_gcell, kernel, log):
         f"({args.served_load} x capacity {capacity:.2f}/s)")
     srv = RTMServer(cfg.dx, cfg.dx, cfg.dt, n_sponge=cfg.n_sponge, k=cfg.k, budget_bytes=200e9,
                     max_active=args.served_slots, kernel=kernel, tick_steps=100)
-    r = run_poisson(srv, cfg.specs, rate, seed=11, make=cfg.make, log=log)
+    srv.set_models(cfg.vel, cfg.vel, cfg.ela, cfg.ela)  # one shared resident model
+    r = run_poisson(srv, cfg.specs, rate, seed=11, make=lambda sp: cfg.make(sp, resident=True),
+                    log=log)
     srv.close()
     n_el = sum(s.kind == "elastic" for s in cfg.specs)
     return {"workload": f"{len(cfg.specs)} Poisson-arrival virtual-shot migrations, "
                         f"{len(cfg.specs) - n_el} acoustic / {n_el} elastic, nz {{256,384,512}} "
-                        "x nx {512,1024,2048}, nt 2000-8000, dx 5 m, dt 0.4 ms, 200 GB budget",
+                        "x nx {512,1024,2048}, nt 2000-8000, dx 5 m, dt 0.4 ms, 200 GB budget, "
+                        "windows of one shared resident model",
             "arrival_rate_per_s": round(rate, 3), "load_vs_capacity": args.served_load,
             "capacity_estimate_per_s": round(capacity, 3),
             "p50_latency_s": round(r["p50_latency_s"], 4),

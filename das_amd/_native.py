@@ -63,6 +63,7 @@ SIGNATURES = {
     "tsw_server_cancel": (I, [P, LL]),
     "tsw_server_stats": (I, [P] + [ctypes.POINTER(LL)] * 7 + [ctypes.POINTER(D)]),
     "tsw_server_destroy": (I, [P]),
+    "tsw_server_set_models": (I, [P, I, I, P, P, P, P]),
     "tsw_fwi_create": (I, [I, I, D, D, D, I, D, I, I, I, P, I, ctypes.POINTER(P)]),
     "tsw_fwi_forward": (I, [P, P, P, ctypes.POINTER(D)]),
     "tsw_fwi_backward": (I, [P, P, P, P]),
@@ -71,6 +72,7 @@ SIGNATURES = {
     "tsw_fwi_probe": (I, [P, I, P]),
     "tsw_fwi_destroy": (I, [P]),
     "tsw_xcorr": (I, [P, I, I, P, I, P, I, I, I, P]),
+    "tsw_flood_columns": (I, [P, I, I, I, P, I, D, P]),
     "tsw_xcorr_device": (I, [P, I, I, P, I, P, I, I, I, P, P]),
     "tsw_time_derivative": (I, [P, I, I, D, I, P]),
     "tsw_green_branch": (I, [P, I, I, I, I, P, ctypes.POINTER(I)]),

@@ -25,15 +25,22 @@ def fill_desc(d, s, keep):
         raise ValueError("source / data length must equal the shot's nt")
     keep += [src, data, rix]
     if getattr(s, "kind", "acoustic") == "elastic":
-        mf = [N.f32(a) for a in s.model_fwd]
-        mb = [N.f32(a) for a in s.model_bwd]
-        if len(mf) != 4 or len(mb) != 4 or any(a.shape != (s.nz, s.nx) for a in mf + mb):
-            raise ValueError("elastic model shape does not match the shot grid")
-        keep += mf + mb
-        d.kind = N.SHOT_ELASTIC
-        for j in range(4):
-            d.ela_fwd[j] = N.ptr(mf[j])
-            d.ela_bwd[j] = N.ptr(mb[j])
+        if s.model_fwd is None and s.model_bwd is None:
+            d.kind = N.SHOT_ELASTIC  # resident models of a server: window at (z0, x0)
+            mf = mb = None
+        else:
+            mf = [N.f32(a) for a in s.model_fwd]
+            mb = [N.f32(a) for a in s.model_bwd]
+        if mf is not None:
+            if len(mf) != 4 or len(mb) != 4 or any(a.shape != (s.nz, s.nx) for a in mf + mb):
+                raise ValueError("elastic model shape does not match the shot grid")
+            keep += mf + mb
+            d.kind = N.SHOT_ELASTIC
+            for j in range(4):
+                d.ela_fwd[j] = N.ptr(mf[j])
+                d.ela_bwd[j] = N.ptr(mb[j])
+    elif s.vel_fwd is None and s.vel_bwd is None:
+        d.kind = N.SHOT_ACOUSTIC  # resident models of a server: window at (z0, x0)
     else:
         vf, vb = N.f32(s.vel_fwd), N.f32(s.vel_bwd)
         if vf.shape != (s.nz, s.nx) or vb.shape != (s.nz, s.nx):

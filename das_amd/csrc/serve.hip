@@ -46,6 +46,8 @@ struct Req {
   long long need = 0;     // device bytes while running
   long long keep = 0;     // device bytes kept after completion (image + illumination)
   int slot = -1;
+  bool resident = false;  // models are the window [z0, z0+nz) x [x0, x0+nx) of the server's
+  int x0 = 0, z0 = 0;     //   resident models, copied on the device at admission
   int cursor = 0;         // acoustic: next block; elastic: unused (global step based)
   int e_start = 0;        // elastic: global elastic step of the first forward step
   char* heavy = nullptr;       // arena block: state, models, data, snapshots
@@ -68,6 +70,11 @@ struct Server {
   DevBuf d_table, d_tbshots, d_eshots, d_tbent, d_tbpre, d_elist;
   Arena arena;
   void* arena_mem = nullptr;
+  // resident models shared by requests (0/1 acoustic fwd/bwd velocity,
+  // 2..5 / 6..9 elastic fwd / bwd lambda, mu, buoyancy_x, buoyancy_z)
+  DevBuf res[10];
+  int res_nx = 0, res_nz = 0;
+  bool res_acoustic = false, res_elastic = false;
   std::vector<TbShot> tbs;
   std::vector<EShot> es;
   // state
@@ -145,9 +152,19 @@ static int admit(Server* S, Req* r, int s) {
     float* coef[2];
     for (int j = 0; j < 2; ++j) {
       coef[j] = (float*)take(nb);
-      float* vel = (float*)up(r->model[j].data(), (size_t)n * sizeof(float));
-      launch_coef(vel, coef[j], n, dtf * dtf, st);  // `vel` space is reused below
-      cur = (char*)vel;
+      if (r->resident) {
+        float* vel = (float*)take(nb);
+        TSW_HIP(hipMemcpy2DAsync(vel, (size_t)r->nx * sizeof(float),
+                                 S->res[j].f() + (size_t)r->z0 * S->res_nx + r->x0,
+                                 (size_t)S->res_nx * sizeof(float), (size_t)r->nx * sizeof(float),
+                                 r->nz, hipMemcpyDeviceToDevice, st));
+        launch_coef(vel, coef[j], n, dtf * dtf, st);
+        cur = (char*)vel;
+      } else {
+        float* vel = (float*)up(r->model[j].data(), (size_t)n * sizeof(float));
+        launch_coef(vel, coef[j], n, dtf * dtf, st);  // `vel` space is reused below
+        cur = (char*)vel;
+      }
     }
     for (int j = 0; j < 3; ++j) {
       P.set[0][j] = (float*)take(nb);
@@ -177,8 +194,20 @@ static int admit(Server* S, Req* r, int s) {
     EShot E;
     std::memset(&E, 0, sizeof E);
     for (int j = 0; j < 4; ++j) {
-      E.mf[j] = (float*)up(r->model[j].data(), (size_t)n * sizeof(float));
-      E.mb[j] = (float*)up(r->model[4 + j].data(), (size_t)n * sizeof(float));
+      if (r->resident) {
+        for (int b = 0; b < 2; ++b) {
+          float* dst = (float*)take((size_t)n * sizeof(float));
+          TSW_HIP(hipMemcpy2DAsync(dst, (size_t)r->nx * sizeof(float),
+                                   S->res[2 + 4 * b + j].f() + (size_t)r->z0 * S->res_nx + r->x0,
+                                   (size_t)S->res_nx * sizeof(float),
+                                   (size_t)r->nx * sizeof(float), r->nz, hipMemcpyDeviceToDevice,
+                                   st));
+          (b == 0 ? E.mf : E.mb)[j] = dst;
+        }
+      } else {
+        E.mf[j] = (float*)up(r->model[j].data(), (size_t)n * sizeof(float));
+        E.mb[j] = (float*)up(r->model[4 + j].data(), (size_t)n * sizeof(float));
+      }
     }
     for (int j = 0; j < 5; ++j) {
       E.f[j] = (float*)take(nb);
@@ -503,6 +532,43 @@ int tsw_server_create(double dx, double dz, double dt, int n_sponge, double spon
   return kOk;
 }
 
+int tsw_server_set_models(tsw_server_t h, int gnx, int gnz, const float* vel_fwd,
+                          const float* vel_bwd, const float* const* ela_fwd,
+                          const float* const* ela_bwd) {
+  Server* S = reinterpret_cast<Server*>(h);
+  TSW_ARG(S && gnx >= 5 && gnz >= 5, "bad arguments");
+  TSW_ARG((vel_fwd == nullptr) == (vel_bwd == nullptr), "give both acoustic models or neither");
+  TSW_ARG((ela_fwd == nullptr) == (ela_bwd == nullptr), "give both elastic models or neither");
+  if (ela_fwd)
+    for (int j = 0; j < 4; ++j) TSW_ARG(ela_fwd[j] && ela_bwd[j], "elastic model %d missing", j);
+  std::lock_guard<std::mutex> lk(S->mu);
+  bool busy = !S->queue.empty();
+  for (Req* r : S->slots) busy = busy || r;
+  TSW_ARG(!busy, "resident models can only be replaced while the server is idle");
+  const size_t n = (size_t)gnx * gnz, nb = n * sizeof(float);
+  const float* src[10] = {vel_fwd, vel_bwd, nullptr, nullptr, nullptr, nullptr,
+                          nullptr, nullptr, nullptr, nullptr};
+  for (int j = 0; j < 4 && ela_fwd; ++j) {
+    src[2 + j] = ela_fwd[j];
+    src[6 + j] = ela_bwd[j];
+  }
+  S->res_acoustic = S->res_elastic = false;
+  for (int j = 0; j < 10; ++j) {
+    if (!src[j]) {
+      S->res[j].release();
+      continue;
+    }
+    TSW_TRY(S->res[j].ensure(nb));
+    TSW_HIP(hipMemcpyAsync(S->res[j].p, src[j], nb, hipMemcpyHostToDevice, S->sa));
+  }
+  TSW_HIP(hipStreamSynchronize(S->sa));
+  S->res_nx = gnx;
+  S->res_nz = gnz;
+  S->res_acoustic = vel_fwd != nullptr;
+  S->res_elastic = ela_fwd != nullptr;
+  return kOk;
+}
+
 int tsw_server_submit(tsw_server_t h, const tsw_shot_desc* d, long long* id_out) {
   Server* S = reinterpret_cast<Server*>(h);
   TSW_ARG(S && d && id_out, "null argument");
@@ -523,7 +589,25 @@ int tsw_server_submit(tsw_server_t h, const tsw_shot_desc* d, long long* id_out)
   r->riz = d->riz;
   r->nrec = d->nrec;
   const size_t n = (size_t)d->nx * d->nz;
+  bool none = true;
   if (d->kind == TSW_SHOT_ACOUSTIC) {
+    none = !d->vel_fwd && !d->vel_bwd;
+  } else {
+    for (int j = 0; j < 4; ++j) none = none && !d->ela_fwd[j] && !d->ela_bwd[j];
+  }
+  if (none) {  // window of the resident models
+    std::lock_guard<std::mutex> lk(S->mu);
+    const bool have = d->kind == TSW_SHOT_ACOUSTIC ? S->res_acoustic : S->res_elastic;
+    TSW_ARG(have, "request without models, and no resident %s models are set",
+            d->kind == TSW_SHOT_ACOUSTIC ? "acoustic" : "elastic");
+    TSW_ARG(d->x0 >= 0 && d->z0 >= 0 && d->x0 + d->nx <= S->res_nx &&
+                d->z0 + d->nz <= S->res_nz,
+            "window [%d, %d) x [%d, %d) outside the %d x %d resident models", d->z0,
+            d->z0 + d->nz, d->x0, d->x0 + d->nx, S->res_nz, S->res_nx);
+    r->resident = true;
+    r->x0 = d->x0;
+    r->z0 = d->z0;
+  } else if (d->kind == TSW_SHOT_ACOUSTIC) {
     TSW_ARG(d->vel_fwd && d->vel_bwd, "missing velocity models");
     r->model[0].assign(d->vel_fwd, d->vel_fwd + n);
     r->model[1].assign(d->vel_bwd, d->vel_bwd + n);

@@ -42,6 +42,27 @@ class RTMServer:
 
     __del__ = close
 
+    def set_models(self, vel_fwd=None, vel_bwd=None, ela_fwd=None, ela_bwd=None):
+        """Uploads models shared by requests (each (gnz, gnx)); a request
+        whose model fields are None migrates in its window (z0, x0) of them.
+        Only while no request is queued or running."""
+        arrs = [a for a in (vel_fwd, vel_bwd) if a is not None]
+        arrs += [a for m in (ela_fwd, ela_bwd) if m is not None for a in m]
+        if not arrs:
+            raise ValueError("no models given")
+        shape = np.asarray(arrs[0]).shape
+        keep = [N.f32(a) for a in arrs]
+        if any(a.shape != shape or a.ndim != 2 for a in keep):
+            raise ValueError("resident models must all have the same (nz, nx) shape")
+        vf = N.f32(vel_fwd) if vel_fwd is not None else None
+        vb = N.f32(vel_bwd) if vel_bwd is not None else None
+        ef = [N.f32(a) for a in ela_fwd] if ela_fwd is not None else None
+        eb = [N.f32(a) for a in ela_bwd] if ela_bwd is not None else None
+        PF = (N.P * 4)(*[N.ptr(a) for a in ef]) if ef is not None else None
+        PB = (N.P * 4)(*[N.ptr(a) for a in eb]) if eb is not None else None
+        N.check(N.lib().tsw_server_set_models(self._h, shape[1], shape[0], N.ptr(vf), N.ptr(vb),
+                                              PF, PB))
+
     def submit(self, shot) -> int:
         keep = []
         d = N.ShotDesc()

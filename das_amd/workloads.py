@@ -60,7 +60,8 @@ class Shot:
     data: np.ndarray            # (nt, nrec)
     rix: np.ndarray
     riz: int
-    x0: int = 0                 # window offset in the global image (columns)
+    x0: int = 0                 # window offset in the global image / resident model (columns)
+    z0: int = 0                 # window offset (rows)
 
     @property
     def cells(self):
@@ -83,6 +84,7 @@ class ElasticShot:
     riz: int
     x0: int = 0
     kind: str = "elastic"
+    z0: int = 0
 
     @property
     def cells(self):
@@ -338,13 +340,23 @@ class Config4:
         offs = (brng.random(bank_traces) - 0.5) * 2 * 2048 * dx
         self.bank = correlogram_gather(brng, 8000, dt, offs, f0=f0).T.copy()  # (traces, nt)
 
-    def make(self, spec: RequestSpec):
+    def make(self, spec: RequestSpec, resident: bool = False):
+        """The request's arrays.  With ``resident`` the models are left out
+        (None): the server migrates in the window of its resident models
+        (``RTMServer.set_models(self.vel, self.vel, self.ela, self.ela)``)."""
         rng = np.random.default_rng(spec.seed)
         rix = (np.arange(spec.nx // 2) * 2).astype(np.int32)
         pick = rng.choice(self.bank.shape[0], len(rix), replace=len(rix) > self.bank.shape[0])
         data = np.ascontiguousarray(self.bank[pick, :spec.nt].T)
         src, _ = ricker_wavelet(self.f0, self.dt, spec.nt * self.dt)
         win = (slice(0, spec.nz), slice(spec.x0, spec.x0 + spec.nx))
+        if resident:
+            if spec.kind == "elastic":
+                return ElasticShot(nx=spec.nx, nz=spec.nz, nt=spec.nt, model_fwd=None,
+                                   model_bwd=None, src=src, six=spec.six, siz=2, data=data,
+                                   rix=rix, riz=2, x0=spec.x0)
+            return Shot(nx=spec.nx, nz=spec.nz, nt=spec.nt, vel_fwd=None, vel_bwd=None, src=src,
+                        six=spec.six, siz=2, data=data, rix=rix, riz=2, x0=spec.x0)
         if spec.kind == "elastic":
             m = tuple(np.ascontiguousarray(a[win]) for a in self.ela)
             return ElasticShot(nx=spec.nx, nz=spec.nz, nt=spec.nt, model_fwd=m, model_bwd=m,

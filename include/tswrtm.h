@@ -226,6 +226,14 @@ int tsw_server_create(double dx, double dz, double dt, int n_sponge, double spon
                       int free_surface_top, int k, long long budget_bytes, int max_active,
                       int kernel, int tick_steps, tsw_server_t* out);
 /* Copies the request's host arrays and queues it; -3 if it can never fit. */
+/* Models shared by requests, resident in HBM outside the budget: acoustic
+ * forward/backward velocities and/or elastic forward/backward (lambda, mu,
+ * buoyancy_x, buoyancy_z), each (gnz, gnx).  A request whose model pointers
+ * are all null migrates in the window [z0, z0+nz) x [x0, x0+nx) of these,
+ * cropped on the device at admission.  Only while the server is idle. */
+int tsw_server_set_models(tsw_server_t srv, int gnx, int gnz, const float* vel_fwd,
+                          const float* vel_bwd, const float* const* ela_fwd,
+                          const float* const* ela_bwd);
 int tsw_server_submit(tsw_server_t server, const tsw_shot_desc* req, long long* id_out);
 int tsw_server_poll(tsw_server_t server, long long id, int* state, double* t_submit,
                     double* t_admit, double* t_done);
@@ -295,6 +303,14 @@ int tsw_green_branch(const float* in, int nl, int ncol, int zero_idx, int branch
  * step dt_sim, 0 outside the original time range. */
 int tsw_resample_linear(const float* in, int nt_orig, int ncol, double dt_orig, double dt_sim,
                         int target_nt, float* out);
+
+/* One flood model of generate_velocity_flood_models
+ * (tsw_rtm/velocity_flood.py:24-94): out[iz][ix] = base[iz][colmap[ix]]
+ * (0 where colmap is -1), times `factor` when scale != 0, computed in the
+ * base's precision (4: float32 with factor rounded to float32, 8: float64)
+ * and stored as float32. */
+int tsw_flood_columns(const void* base, int precision, int nx, int nz, const int* colmap,
+                      int scale, double factor, float* out);
 
 #ifdef __cplusplus
 }

@@ -145,3 +145,53 @@ def test_budget_rejection_cancel_and_injected_failure(reqs):
     st = srv.stats()
     assert st["failed"] == 1 and st["cancelled"] == 1
     srv.close()
+
+
+def test_resident_model_windows_equal_uploaded_models():
+    """Requests that name a window of the server's resident models give the
+    results of the same requests carrying copies of those windows."""
+    from dataclasses import replace
+
+    from das_amd.wave_solver import ricker_wavelet
+    from das_amd.workloads import (ElasticShot, Shot, correlogram_gather, elastic_from_vp,
+                                   layered_faulted_vp, smooth)
+
+    rng = np.random.default_rng(21)
+    gnz, gnx = 90, 420
+    vp = layered_faulted_vp(gnz, gnx, DX, seed=3)
+    vel = smooth(vp, 30.0, DX)
+    ela = elastic_from_vp(vel)
+    reqs = []
+    for kind, nz, nx, nt, x0, z0 in [("a", 64, 200, 120, 0, 0), ("e", 48, 96, 60, 310, 20),
+                                     ("a", 90, 160, 80, 260, 0), ("e", 40, 120, 70, 7, 50),
+                                     ("a", 50, 256, 90, 164, 33)]:
+        six = int(rng.integers(5, nx - 5))
+        rix = np.arange(1, nx - 1, 3).astype(np.int32)
+        src, _ = ricker_wavelet(25.0, DT, nt * DT)
+        data = correlogram_gather(rng, nt, DT, (rix - six) * DX, f0=25.0, f_lo=5, f_hi=50)
+        win = (slice(z0, z0 + nz), slice(x0, x0 + nx))
+        if kind == "a":
+            v = np.ascontiguousarray(vel[win])
+            reqs.append(Shot(nx=nx, nz=nz, nt=nt, vel_fwd=v, vel_bwd=v, src=src, six=six,
+                             siz=2, data=data, rix=rix, riz=2, x0=x0, z0=z0))
+        else:
+            m = tuple(np.ascontiguousarray(a[win]) for a in ela)
+            reqs.append(ElasticShot(nx=nx, nz=nz, nt=nt, model_fwd=m, model_bwd=m, src=src,
+                                    six=six, siz=2, data=data, rix=rix, riz=2, x0=x0, z0=z0))
+    srv = _server(max_active=3, tick_steps=23)
+    with pytest.raises(ValueError, match="no resident"):
+        srv.submit(replace(reqs[0], vel_fwd=None, vel_bwd=None))
+    srv.set_models(vel, vel, ela, ela)
+    with pytest.raises(ValueError, match="outside"):
+        srv.submit(replace(reqs[0], vel_fwd=None, vel_bwd=None, x0=gnx - 100))
+    up = [srv.submit(s) for s in reqs]
+    res = [srv.submit(replace(s, vel_fwd=None, vel_bwd=None) if isinstance(s, Shot)
+                      else replace(s, model_fwd=None, model_bwd=None)) for s in reqs]
+    for a, b in zip(up, res):
+        assert srv.wait(a)["state"] == "done" and srv.wait(b)["state"] == "done"
+        ia, la = srv.result(a)
+        ib, lb = srv.result(b)
+        np.testing.assert_array_equal(ia, ib)
+        np.testing.assert_array_equal(la, lb)
+        assert np.abs(ia).max() > 0
+    srv.close()
