@@ -73,6 +73,8 @@ SIGNATURES = {
     "tsw_fwi_destroy": (I, [P]),
     "tsw_xcorr": (I, [P, I, I, P, I, P, I, I, I, P]),
     "tsw_flood_columns": (I, [P, I, I, I, P, I, D, P]),
+    "tsw_save_matrix_csv": (I, [ctypes.c_char_p, P, I, I, I, P, P, ctypes.c_char_p,
+                                ctypes.c_char_p]),
     "tsw_xcorr_device": (I, [P, I, I, P, I, P, I, I, I, P, P]),
     "tsw_time_derivative": (I, [P, I, I, D, I, P]),
     "tsw_green_branch": (I, [P, I, I, I, I, P, ctypes.POINTER(I)]),

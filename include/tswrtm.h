@@ -312,6 +312,14 @@ int tsw_resample_linear(const float* in, int nt_orig, int ncol, double dt_orig, 
 int tsw_flood_columns(const void* base, int precision, int nx, int nz, const int* colmap,
                       int scale, double factor, float* out);
 
+/* save_matrix_csv (tsw_rtm/visualizer.py:19-39), byte for byte: header
+ * "row_name/col_name," + column axis (%.6g), then per row its axis value
+ * (%.6g) and the values (%.6e).  matrix: (nr, nc) float32 (precision 4) or
+ * float64 (8); axes float64.  Host only. */
+int tsw_save_matrix_csv(const char* path, const void* matrix, int precision, int nr, int nc,
+                        const double* row_axis, const double* col_axis, const char* row_name,
+                        const char* col_name);
+
 #ifdef __cplusplus
 }
 #endif
