@@ -67,6 +67,7 @@ SIGNATURES = {
     "tsw_fwi_create": (I, [I, I, D, D, D, I, D, I, I, I, P, I, ctypes.POINTER(P)]),
     "tsw_fwi_forward": (I, [P, P, P, ctypes.POINTER(D)]),
     "tsw_fwi_backward": (I, [P, P, P, P]),
+    "tsw_fwi_set_blocking": (I, [P, I]),
     "tsw_fwi_predicted": (I, [P, I, P]),
     "tsw_fwi_set_residual": (I, [P, I, P]),
     "tsw_fwi_probe": (I, [P, I, P]),

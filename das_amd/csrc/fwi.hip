@@ -28,10 +28,12 @@
 // recomputes each segment from its checkpoint (bit-identical to the first
 // pass) before running that segment's adjoint steps.
 #include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
 #include "engine.h"
+#include "tb.h"
 #include "tswrtm.h"
 
 namespace tsw {
@@ -182,7 +184,9 @@ struct AdjArgs {
   T rdx2, rdz2;
   long long n;
   const T* C;
-  const T* S;
+  const T* S;            // sponge per cell (used when `table` is null)
+  const float* table;    // FP32: sponge factor by depth, computed per cell instead of read
+  int n_sponge;
   const T* lam1;  // Lam^{n+1}
   const T* lam2;  // Lam^{n+2}
   const T* lam3;  // Lam^{n+3}
@@ -210,18 +214,26 @@ __device__ __forceinline__ T fwi_zeta(int fs, const T* lam, int c, int iz, int n
 }
 
 template <typename T>
+__device__ __forceinline__ T fwi_sponge(const AdjArgs<T>& a, int iz, int ix, int c) {
+  if (a.table == nullptr) return a.S[c];
+  const int d = sponge_depth(ix, iz, a.nx, a.nz, a.n_sponge, a.fs);
+  return d > 0 ? (T)a.table[d] : T(1);
+}
+
+template <typename T, int TZ>
 __global__ __launch_bounds__(256) void k_fwi_adj(AdjArgs<T> a) {
-  __shared__ T cw[kLH * kLW];  // C.w^{n+1}, zero outside I
+  constexpr int LH = TZ + 4;
+  __shared__ T cw[LH * kLW];  // C.w^{n+1}, zero outside I
   const int nx = a.nx, nz = a.nz;
-  const int x0 = blockIdx.x * 64, z0 = blockIdx.y * kTZ;
+  const int x0 = blockIdx.x * 64, z0 = blockIdx.y * TZ;
   const long long o = (long long)blockIdx.z * a.n;
   const T* lam1 = a.lam1 + o;
-  for (int i = threadIdx.y * 64 + threadIdx.x; i < kLH * kLW; i += 256) {
+  for (int i = threadIdx.y * 64 + threadIdx.x; i < LH * kLW; i += 256) {
     const int gz = z0 - 2 + i / kLW, gx = x0 - 2 + i % kLW;
     T v = T(0);
     if (fwi_interior<T>(gz, gx, nz, nx)) {
       const int c = gz * nx + gx;
-      v = a.C[c] * (a.S[c] * fwi_zeta(a.fs, lam1, c, gz, nx));
+      v = a.C[c] * (fwi_sponge(a, gz, gx, c) * fwi_zeta(a.fs, lam1, c, gz, nx));
     }
     cw[i] = v;
   }
@@ -230,7 +242,7 @@ __global__ __launch_bounds__(256) void k_fwi_adj(AdjArgs<T> a) {
   if (ix >= nx) return;
   const FwiMeta& m = a.meta[blockIdx.z];
   const T c0 = FwiK<T>::c0, c1 = fwi_c1<T>(), c2 = fwi_c2<T>();
-  for (int k = 0; k < kTZ / 4; ++k) {
+  for (int k = 0; k < TZ / 4; ++k) {
     const int lz = threadIdx.y + 4 * k, iz = z0 + lz;
     if (iz >= nz) break;
     const int c = iz * nx + ix;
@@ -244,7 +256,7 @@ __global__ __launch_bounds__(256) void k_fwi_adj(AdjArgs<T> a) {
     lz2 = (lz2 + c0 * cw[l]) * a.rdz2;
     T acc;
     if (fwi_interior<T>(iz, ix, nz, nx)) {
-      const T sc = a.S[c];
+      const T sc = fwi_sponge(a, iz, ix, c);
       const T w1 = sc * fwi_zeta(a.fs, lam1, c, iz, nx);
       const T w2 = sc * fwi_zeta(a.fs, a.lam2 + o, c, iz, nx);
       acc = (T(2) * w1 + (lx + lz2)) - w2;
@@ -342,6 +354,180 @@ __global__ void k_fwi_loss(const double* partial, double* loss) {
   if (threadIdx.x == 0) *loss = 0.5 * red[0];
 }
 
+// ---- Temporally blocked forward steps (FP32).  One launch advances every
+// shot of a group by up to T steps.  A workgroup owns a core tile and loads it
+// with a 2T-cell halo (clipped at the grid); each thread keeps its R rows'
+// three time levels, coefficient and sponge factor in registers and exchanges
+// neighbours through double-buffered LDS, one barrier per step.  After s steps
+// the cells at least 2s from a cut tile edge are exact, so the core is exact
+// after T steps.  The per-cell arithmetic is that of k_fwi_fwd<float> (the
+// production FP32 association), so results are bit-identical to it.  The state
+// at a block boundary (u^{n-2}, u^{n-1}, u^n) ping-pongs between two sets.
+struct FwiTbArgs {
+  int nx, nz, fs, n_sponge;
+  float rdx2, rdz2, inj;
+  const float* C;
+  const float* table;        // sponge factor by depth
+  long long n;               // cells per shot
+  const float* rd[3];        // set read: u^{n-2}, u^{n-1}, u^n (shot j at + j * n)
+  float* wr[3];              // set written
+  float* save;               // non-null: save + (it - save_it0) * sstride + j * n = u^it
+  long long sstride;
+  int save_it0;
+  const FwiMeta* meta;
+  int it0, nsub, nt, rec;    // rec: record the receivers (P^{it+1}, before injection)
+  int tiles_x, tiles;        // tiles per shot
+  int count;                 // shots: the grid is tiles * count workgroups
+};
+
+template <int T, int R, int WX, int WZ>
+__global__ __launch_bounds__(64 * WX * WZ) void k_fwi_tb(FwiTbArgs a) {
+  using S = TbShape<T, R, WX, WZ>;
+  __shared__ float lds[2][S::LR * S::LS];
+  const int b = xcd_tile_tb(blockIdx.x, a.tiles * a.count);
+  const int shot = b / a.tiles, t = b - shot * a.tiles;
+  const FwiMeta& m = a.meta[shot];
+  const long long o = (long long)shot * a.n;
+  const int nx = a.nx, nz = a.nz;
+  const int cx0 = (t % a.tiles_x) * S::W;
+  const int cz0 = (t / a.tiles_x) * S::H;
+  const int X0 = cx0 - 2 * T > 0 ? cx0 - 2 * T : 0;
+  const int Z0 = cz0 - 2 * T > 0 ? cz0 - 2 * T : 0;
+  const int lx = threadIdx.x, ly = threadIdx.y;
+  const int col = X0 + lx;
+  const int row0 = Z0 + ly * R;
+  const bool col_in = col < nx;
+  const bool core_col = col_in && col >= cx0 && col < cx0 + S::W;
+  const bool col_int = col >= 2 && col < nx - 2;
+  unsigned m_in = 0, m_int = 0, m_core = 0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int row = row0 + q;
+    if (col_in && row < nz) {
+      m_in |= 1u << q;
+      if (col_int && row >= 2 && row < nz - 2) m_int |= 1u << q;
+      if (core_col && row >= cz0 && row < cz0 + S::H) m_core |= 1u << q;
+    }
+  }
+  const unsigned m_fs = (a.fs && row0 == 0) ? (m_in & 3u) : 0u;
+  const unsigned m_keep = m_in & ~m_int & ~m_fs;
+  float u2[R], u1[R], c[R], cf[R], sp[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    u2[q] = u1[q] = c[q] = cf[q] = 0.0f;
+    sp[q] = 1.0f;
+    if (m_in >> q & 1u) {
+      const int idx = (row0 + q) * nx + col;
+      cf[q] = a.C[idx];
+      const int d = sponge_depth(col, row0 + q, nx, nz, a.n_sponge, a.fs);
+      if (d > 0) sp[q] = a.table[d];
+      c[q] = a.rd[2][o + idx];
+      u1[q] = a.rd[1][o + idx];
+      if (m_keep >> q & 1u) u2[q] = a.rd[0][o + idx];
+    }
+  }
+  float* X = lds[0];
+#pragma unroll
+  for (int q = 0; q < R; ++q) X[(ly * R + q + 2) * S::LS + lx + 2] = c[q];
+  __syncthreads();
+  const int src_row = m.src_cell / nx, src_col = m.src_cell - src_row * nx;
+  for (int j = 0; j < a.nsub; ++j) {
+    const int it = a.it0 + j;
+    float* Y = lds[(j + 1) & 1];
+    if (a.save && m_core) {
+      float* sv = a.save + (long long)(it - a.save_it0) * a.sstride + o;
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+        if (m_core >> q & 1u) sv[(row0 + q) * nx + col] = c[q];
+    }
+    float nv[R];
+    const float* xr = X + 2;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int lr = (ly * R + q + 2) * S::LS + lx;
+      float v;
+      if (m_int >> q & 1u) {
+        const float pc = c[q];
+        const float xm2 = xr[lr - 2], xm1 = xr[lr - 1], xp1 = xr[lr + 1], xp2 = xr[lr + 2];
+        const float zm2 = q >= 2 ? c[q - 2] : xr[lr - 2 * S::LS];
+        const float zm1 = q >= 1 ? c[q - 1] : xr[lr - S::LS];
+        const float zp1 = q + 1 < R ? c[q + 1] : xr[lr + S::LS];
+        const float zp2 = q + 2 < R ? c[q + 2] : xr[lr + 2 * S::LS];
+        float lxx = TSW_C2 * (xm2 + xp2);
+        lxx = lxx + TSW_C1 * (xm1 + xp1);
+        lxx = (lxx + TSW_C0 * pc) * a.rdx2;
+        float lzz = TSW_C2 * (zm2 + zp2);
+        lzz = lzz + TSW_C1 * (zm1 + zp1);
+        lzz = (lzz + TSW_C0 * pc) * a.rdz2;
+        v = (2.0f * pc - u1[q]) + cf[q] * (lxx + lzz);
+        v = v * sp[q];
+      } else {
+        v = u2[q];  // cells the step never writes keep the level three buffers back
+      }
+      nv[q] = v;
+    }
+    if (m_fs) {  // rows 0, 1, 2 of this column belong to this thread
+      nv[0] = 0.0f;
+      nv[1] = -nv[2];
+    }
+    if (a.rec && m.riz >= row0 && m.riz < row0 + R && core_col) {
+      const int r0 = m.roff[col], r1 = m.roff[col + 1];
+      if (r0 < r1) {
+        float* rec = (float*)m.pred + (long long)it * m.nrec;
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          if (row0 + q == m.riz && (m_core >> q & 1u))
+            for (int r = r0; r < r1; ++r) rec[m.ridx[r]] = nv[q];
+      }
+    }
+    if (it + 1 < a.nt && src_col == col && src_row >= row0 && src_row < row0 + R) {
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+        if (row0 + q == src_row) nv[q] = nv[q] + ((const float*)m.src)[it + 1] * a.inj;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      Y[(ly * R + q + 2) * S::LS + lx + 2] = nv[q];
+      u2[q] = u1[q];
+      u1[q] = c[q];
+      c[q] = nv[q];
+    }
+    X = Y;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    if (m_core >> q & 1u) {
+      const int idx = (row0 + q) * nx + col;
+      a.wr[2][o + idx] = c[q];
+      a.wr[1][o + idx] = u1[q];
+      a.wr[0][o + idx] = u2[q];
+    }
+  }
+}
+
+// Blocked configurations: steps per launch T, rows per thread R, wavefronts
+// across WX and down WZ.
+struct FwiTbCfg {
+  int T, R, WX, WZ;
+  void (*launch)(const FwiTbArgs&, int count, hipStream_t);
+};
+template <int T, int R, int WX, int WZ>
+static void fwi_tb_launch(const FwiTbArgs& a, int count, hipStream_t st) {
+  (void)count;
+  hipLaunchKernelGGL((k_fwi_tb<T, R, WX, WZ>), dim3(a.tiles * a.count), dim3(64 * WX, WZ), 0,
+                     st, a);
+}
+static const FwiTbCfg kFwiTb[] = {
+    {4, 4, 1, 16, fwi_tb_launch<4, 4, 1, 16>}, {5, 4, 1, 16, fwi_tb_launch<5, 4, 1, 16>},
+    {4, 8, 1, 8, fwi_tb_launch<4, 8, 1, 8>},   {4, 16, 1, 4, fwi_tb_launch<4, 16, 1, 4>},
+    {3, 4, 1, 16, fwi_tb_launch<3, 4, 1, 16>}, {5, 8, 1, 8, fwi_tb_launch<5, 8, 1, 8>},
+    {3, 8, 1, 8, fwi_tb_launch<3, 8, 1, 8>},   {4, 12, 1, 8, fwi_tb_launch<4, 12, 1, 8>},
+    {6, 8, 1, 8, fwi_tb_launch<6, 8, 1, 8>},
+};
+constexpr int kFwiTbN = (int)(sizeof(kFwiTb) / sizeof(kFwiTb[0]));
+constexpr int kFwiDefaultTb = 3;
+
 struct FwiShotHost {
   int nt, six, siz, riz, nrec;
   DevBuf src, obs, pred, res, roff, ridx, probe;
@@ -362,6 +548,8 @@ struct Fwi {
   std::vector<FwiShotHost> shots;
   std::vector<FwiGroup> groups;
   DevBuf S, C, buf, mu, seg, gC, partial, loss;
+  int tb = 0;          // FP32 blocked forward: configuration kFwiTb[tb - 1]; 0 = per step
+  DevBuf sets, table;  // blocked forward: two state sets [2][3][cmax][n], sponge table
   double last_loss = 0;
   bool have_forward = false;
   size_t esz() const { return prec == 8 ? 8 : 4; }
@@ -423,6 +611,72 @@ static void fwd_step(const Fwi* F, const FwiGroup& G, FwdArgs<T>& s, int it, T* 
                      dim3(64, 4), 0, st, s);
 }
 
+// level l (0: u^{n-2}, 1: u^{n-1}, 2: u^n) of blocked-forward state set k
+static float* tb_set(const Fwi* F, int k, int l) {
+  return (float*)F->sets.p + (long long)(k * 3 + l) * F->cmax * F->cells();
+}
+
+static FwiTbArgs tb_base(const Fwi* F, const FwiGroup& G) {
+  FwiTbArgs a;
+  std::memset(&a, 0, sizeof a);
+  const float dxf = (float)F->dx, dzf = (float)F->dz;
+  a.nx = F->nx;
+  a.nz = F->nz;
+  a.fs = F->fs;
+  a.n_sponge = F->n_sponge;
+  a.rdx2 = 1.0f / (dxf * dxf);
+  a.rdz2 = 1.0f / (dzf * dzf);
+  a.inj = (float)(F->dt * F->dt);
+  a.C = (const float*)F->C.p;
+  a.table = F->table.f();
+  a.n = F->cells();
+  a.meta = (const FwiMeta*)G.meta.p;
+  a.nt = G.nt;
+  a.count = G.count();
+  const FwiTbCfg& c = kFwiTb[F->tb - 1];
+  const int W = 64 * c.WX - 4 * c.T, H = c.WZ * c.R - 4 * c.T;
+  a.tiles_x = (F->nx + W - 1) / W;
+  a.tiles = a.tiles_x * ((F->nz + H - 1) / H);
+  return a;
+}
+
+// Blocked steps [i0, i1) of a group from state set *k (updated), recording
+// the receivers or saving every level u^it into `save` (segment starting at
+// save_it0); blocks never cross a multiple of `cut` (checkpoint boundaries,
+// 0: none) so that the state is in memory there.
+static void tb_run(const Fwi* F, const FwiGroup& G, int i0, int i1, int* k, bool rec, float* save,
+                   int save_it0, hipStream_t st) {
+  FwiTbArgs a = tb_base(F, G);
+  const FwiTbCfg& c = kFwiTb[F->tb - 1];
+  a.rec = rec ? 1 : 0;
+  a.save = save;
+  a.save_it0 = save_it0;
+  a.sstride = (long long)G.count() * F->cells();
+  for (int it = i0; it < i1;) {
+    const int len = std::min(c.T, i1 - it);
+    for (int l = 0; l < 3; ++l) {
+      a.rd[l] = tb_set(F, *k, l);
+      a.wr[l] = tb_set(F, *k ^ 1, l);
+    }
+    a.it0 = it;
+    a.nsub = len;
+    c.launch(a, G.count(), st);
+    *k ^= 1;
+    it += len;
+  }
+}
+
+// state set k := (0, 0, src[0] * inj at the source cell) for every shot
+static void tb_reset(const Fwi* F, const FwiGroup& G, int k, hipStream_t st) {
+  const long long n = F->cells(), cn = (long long)G.count() * n;
+  for (int l = 0; l < 3; ++l)
+    hipLaunchKernelGGL(k_fwi_fill<float>, dim3(g1(cn)), dim3(256), 0, st, tb_set(F, k, l), cn,
+                       0.0f);
+  hipLaunchKernelGGL(k_fwi_inject0<float>, dim3((G.count() + 63) / 64), dim3(64), 0, st,
+                     tb_set(F, k, 2), n, (const FwiMeta*)G.meta.p, G.count(),
+                     (float)(F->dt * F->dt));
+}
+
 template <typename T>
 static int fwi_forward_t(Fwi* F, const T* d_vel, hipStream_t st, bool store_all) {
   const long long n = F->cells();
@@ -432,6 +686,22 @@ static int fwi_forward_t(Fwi* F, const T* d_vel, hipStream_t st, bool store_all)
   const int K = F->kint;
   for (auto& G : F->groups) {
     const long long cn = (long long)G.count() * n;
+    if (std::is_same<T, float>::value && F->tb > 0) {
+      int k = 0;
+      tb_reset(F, G, k, st);
+      for (int it = 0; it < G.nt;) {
+        int end = G.nt;
+        if (!store_all) {
+          T* ck = (T*)G.ckpt.p + (long long)(it / K) * 3 * cn;  // it % K == 0 here
+          for (int l = 0; l < 3; ++l)
+            TSW_HIP(hipMemcpyAsync(ck + l * cn, tb_set(F, k, l), cn * sizeof(T),
+                                   hipMemcpyDeviceToDevice, st));
+          end = std::min(G.nt, it + K);
+        }
+        tb_run(F, G, it, end, &k, true, nullptr, 0, st);
+        it = end;
+      }
+    } else {
     fwd_reset<T>(F, G, st);
     FwdArgs<T> s = fwd_base<T>(F, G);
     for (int it = 0; it < G.nt; ++it) {
@@ -445,6 +715,7 @@ static int fwi_forward_t(Fwi* F, const T* d_vel, hipStream_t st, bool store_all)
                                hipMemcpyDeviceToDevice, st));
       }
       fwd_step<T>(F, G, s, it, nullptr, true, st);
+    }
     }
     for (int j = 0; j < G.count(); ++j) {
       const int id = G.ids[j];
@@ -494,6 +765,10 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
     a.n = n;
     a.C = (const T*)F->C.p;
     a.S = (const T*)F->S.p;
+    if (std::is_same<T, float>::value && F->table.p) {
+      a.table = F->table.f();
+      a.n_sponge = F->n_sponge;
+    }
     a.grad = (T*)F->gC.p;
     a.meta = (const FwiMeta*)G.meta.p;
     FwdArgs<T> s = fwd_base<T>(F, G);
@@ -501,6 +776,18 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
     const int nseg = (nt + K - 1) / K;
     for (int sg = nseg - 1; sg >= 0; --sg) {
       const int n0 = sg * K, n1 = std::min(nt, n0 + K);
+      if (std::is_same<T, float>::value && F->tb > 0) {
+        int k = 0;
+        if (store_all) {
+          tb_reset(F, G, k, st);
+        } else {
+          const T* ck = (const T*)G.ckpt.p + (long long)sg * 3 * cn;
+          for (int l = 0; l < 3; ++l)
+            TSW_HIP(hipMemcpyAsync(tb_set(F, k, l), ck + l * cn, cn * sizeof(T),
+                                   hipMemcpyDeviceToDevice, st));
+        }
+        tb_run(F, G, n0, n1, &k, false, (float*)F->seg.p, n0, st);
+      } else {
       if (store_all) {
         fwd_reset<T>(F, G, st);  // one segment: the whole forward again
       } else {
@@ -515,6 +802,7 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
       }
       for (int it = n0; it < n1; ++it)
         fwd_step<T>(F, G, s, it, (T*)F->seg.p + (long long)(it - n0) * cn, false, st);
+      }
       for (int it = n1 - 1; it >= n0; --it) {
         a.lam0 = adj<T>(F, it % 4);
         a.lam1 = adj<T>(F, (it + 1) % 4);
@@ -523,7 +811,7 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
         a.un = (const T*)F->seg.p + (long long)(it - n0) * cn;
         a.res_it = it >= 1 ? it - 1 : -1;
         a.probe_it = it;
-        hipLaunchKernelGGL(k_fwi_adj<T>, dim3((F->nx + 63) / 64, (F->nz + kTZ - 1) / kTZ, cnt),
+        hipLaunchKernelGGL((k_fwi_adj<T, 16>), dim3((F->nx + 63) / 64, (F->nz + 15) / 16, cnt),
                            dim3(64, 4), 0, st, a);
       }
     }
@@ -666,6 +954,15 @@ int tsw_fwi_create(int nx, int nz, double dx, double dz, double dt, int n_sponge
       (rc = F->loss.ensure(nshot * sizeof(double))))
     return fail(rc);
   F->kint = store_all ? -1 : F->kint;
+  if (precision == 4) {  // blocked forward by default in FP32
+    std::vector<float> tab;
+    sponge_table(tab, n_sponge, (float)sponge_alpha);
+    if ((rc = F->sets.ensure(6 * cn)) || (rc = F->table.ensure(tab.size() * sizeof(float))))
+      return fail(rc);
+    if (hipMemcpy(F->table.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice))
+      return fail((set_error("sponge table upload failed"), kErrHip));
+    F->tb = kFwiDefaultTb;
+  }
   *out = reinterpret_cast<tsw_fwi_t>(F);
   return kOk;
 }
@@ -692,6 +989,17 @@ int tsw_fwi_backward(tsw_fwi_t h, const void* d_vel, void* d_grad, void* stream)
                                                store_all)
                       : fwi_backward_t<float>(F, (const float*)d_vel, (float*)d_grad, st,
                                               store_all);
+}
+
+int tsw_fwi_set_blocking(tsw_fwi_t h, int config) {
+  Fwi* F = reinterpret_cast<Fwi*>(h);
+  TSW_ARG(F, "null argument");
+  TSW_ARG(config >= 0 && config <= kFwiTbN, "blocking configuration %d outside [0, %d]", config,
+          kFwiTbN);
+  TSW_ARG(config == 0 || F->prec == 4, "the blocked forward is FP32 only");
+  F->tb = config;
+  F->have_forward = false;  // checkpoints of another path: run the forward again
+  return kOk;
 }
 
 int tsw_fwi_predicted(tsw_fwi_t h, int shot, void* host_out) {

@@ -125,6 +125,13 @@ class WaveformMisfit:
                                          ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(stream)))
         return g
 
+    def set_blocking(self, config: int):
+        """FP32 forward and recompute: 0 = one launch per step, 1..6 = a
+        temporally blocked configuration (default 3); results are
+        bit-identical either way."""
+        N.check(N.lib().tsw_fwi_set_blocking(self._h, int(config)))
+        self._last_vel = None
+
     def predicted(self, shot: int) -> np.ndarray:
         """Predicted data (nt, nrec) of a shot from the last forward."""
         nt, nrec = self.shots[shot]

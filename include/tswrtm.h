@@ -265,6 +265,9 @@ int tsw_fwi_create(int nx, int nz, double dx, double dz, double dt, int n_sponge
 int tsw_fwi_forward(tsw_fwi_t fwi, const void* d_vel, void* stream, double* loss_out);
 /* dJ/dv into d_grad (adjoint sweep with recompute); needs tsw_fwi_forward first. */
 int tsw_fwi_backward(tsw_fwi_t fwi, const void* d_vel, void* d_grad, void* stream);
+/* FP32 forward and recompute: 0 = one launch per step, c = 1..6 = temporally
+ * blocked configuration c = 1..9 (the default is 3).  FP64 runs one launch per step. */
+int tsw_fwi_set_blocking(tsw_fwi_t fwi, int config);
 /* Predicted data of a shot from the last forward, (nt, nrec), to the host. */
 int tsw_fwi_predicted(tsw_fwi_t fwi, int shot, void* host_out);
 /* Overwrites a shot's residual (nt, nrec) before tsw_fwi_backward (adjoint tests). */

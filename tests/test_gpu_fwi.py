@@ -259,3 +259,34 @@ def test_autograd_function():
     torch.testing.assert_close(w.grad, op.gradient(w.detach()), rtol=0, atol=0)
     with pytest.raises(ValueError):
         op(_dev(v0, np.float32))
+
+
+@pytest.mark.parametrize("config", [1, 2, 3, 4, 5, 6, 7, 8, 9])
+def test_blocked_forward_equals_per_step(config):
+    """Temporally blocked FP32 forward and recompute: data, misfit and
+    gradient bit-identical to one launch per step, with checkpoints and with
+    every level kept, for shots of two lengths."""
+    from das_amd.fwi import MisfitShot
+
+    v_true, v0 = _models()
+    shots = _shots(v_true, n=3, dtype=np.float32)
+    s1 = shots[1]
+    shots[1] = MisfitShot(src=s1.src[:NT - 37], six=s1.six, siz=s1.siz, rix=s1.rix, riz=s1.riz,
+                          d_obs=s1.d_obs[:NT - 37])
+    v = _dev(v0, np.float32)
+    for ck in (0, -1, 7):
+        ref = _op(shots, np.float32, ckpt=ck)
+        ref.set_blocking(0)
+        lr = ref.forward_loss(v)
+        gr = ref.gradient(v).cpu().numpy()
+        op = _op(shots, np.float32, ckpt=ck)
+        op.set_blocking(config)
+        lb = op.forward_loss(v)
+        for i in range(3):
+            np.testing.assert_array_equal(op.predicted(i), ref.predicted(i))
+        assert lb == lr
+        np.testing.assert_array_equal(op.gradient(v).cpu().numpy(), gr)
+        op.close()
+        ref.close()
+    with pytest.raises(ValueError):
+        _op(shots[:1], np.float64).set_blocking(1)
