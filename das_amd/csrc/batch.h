@@ -3,6 +3,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "tswrtm.h"
 
 namespace tsw {
 
@@ -37,6 +38,15 @@ struct Epoch {
   int tiles;          // total tiles
   int* d_list;        // device: [nshot] shot ids then [nshot+1] tile prefix
 };
+
+// Blocked kernel configuration used when a caller does not choose one (the
+// autotuner's choice for 300-cell-deep windows on MI355X).
+constexpr int kDefaultBlockedKernel = 15;
+
+// Stacks every shot's image and illumination of batch `h` into d_img / d_ill
+// (gnz, gnx), in shot order at the shots' window offsets, on stream `st`.
+int batch_stack_device(tsw_batch_t h, int gnx, int gnz, float* d_img, float* d_ill,
+                       hipStream_t st);
 
 // Constants shared by every shot of a batch.
 struct BatchConst {

@@ -764,14 +764,37 @@ int tsw_batch_shot_result(tsw_batch_t h, int shot, float* image_out, float* illu
   return kOk;
 }
 
-int tsw_batch_stack(tsw_batch_t h, int gnx, int gnz, float* image_out, float* illum_out) {
+}  // extern "C"
+
+namespace tsw {
+int batch_stack_device(tsw_batch_t h, int gnx, int gnz, float* d_img, float* d_ill,
+                       hipStream_t st) {
   Batch* B = reinterpret_cast<Batch*>(h);
-  TSW_ARG(B && gnx >= 1 && gnz >= 1 && image_out && illum_out, "bad argument");
+  TSW_ARG(B && gnx >= 1 && gnz >= 1 && d_img && d_ill, "bad argument");
   for (int i = 0; i < B->ntotal; ++i) {
     const ShotHost& s = shot_host(B, i);
     TSW_ARG(s.x0 >= 0 && s.z0 >= 0 && s.x0 + s.nx <= gnx && s.z0 + s.nz <= gnz,
             "shot window outside the %d x %d stack", gnx, gnz);
   }
+  const long long n = (long long)gnx * gnz;
+  launch_fill(d_img, n, 0.0f, st);
+  launch_fill(d_ill, n, 0.0f, st);
+  for (int i = 0; i < B->ntotal; ++i) {  // shot order, as the reference's stack loop
+    const ShotHost& s = shot_host(B, i);
+    launch_stack_window(d_img + (long long)s.z0 * gnx, gnx, shot_image(B, i), s.nx, s.nz, s.x0, 0,
+                        st);
+    launch_stack_window(d_ill + (long long)s.z0 * gnx, gnx, shot_illum(B, i), s.nx, s.nz, s.x0, 0,
+                        st);
+  }
+  TSW_HIP(hipGetLastError());
+  return kOk;
+}
+}  // namespace tsw
+
+extern "C" {
+
+int tsw_batch_stack(tsw_batch_t h, int gnx, int gnz, float* image_out, float* illum_out) {
+  TSW_ARG(image_out && illum_out, "bad argument");
   Ctx* ctx;
   TSW_TRY(ctx_get(&ctx));
   hipStream_t st = ctx->stream;
@@ -779,16 +802,7 @@ int tsw_batch_stack(tsw_batch_t h, int gnx, int gnz, float* image_out, float* il
   DevBuf img, ill;
   TSW_TRY(img.ensure(n * sizeof(float)));
   TSW_TRY(ill.ensure(n * sizeof(float)));
-  launch_fill(img.f(), n, 0.0f, st);
-  launch_fill(ill.f(), n, 0.0f, st);
-  for (int i = 0; i < B->ntotal; ++i) {  // shot order, as the reference's stack loop
-    const ShotHost& s = shot_host(B, i);
-    launch_stack_window(img.f() + (long long)s.z0 * gnx, gnx, shot_image(B, i), s.nx, s.nz, s.x0,
-                        0, st);
-    launch_stack_window(ill.f() + (long long)s.z0 * gnx, gnx, shot_illum(B, i), s.nx, s.nz, s.x0,
-                        0, st);
-  }
-  TSW_HIP(hipGetLastError());
+  TSW_TRY(batch_stack_device(h, gnx, gnz, img.f(), ill.f(), st));
   TSW_HIP(hipMemcpyAsync(image_out, img.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
   TSW_HIP(hipMemcpyAsync(illum_out, ill.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
   TSW_HIP(hipStreamSynchronize(st));

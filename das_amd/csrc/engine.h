@@ -63,6 +63,7 @@ struct Grid {
   DevBuf sponge, table, buf[3], coef[2], vel, roff, ridx, src, data, rec, snaps, illum, image,
       isnap;
   DevBuf ef[5], epar[2][4];  // elastic state (vx, vz, sxx, szz, sxz) and models
+  bool custom_sponge = false;  // the caller replaced the sponge (tsw_grid_set_sponge)
   long long cells() const { return (long long)nx * nz; }
   AcuStep base() const;  // geometry + constants, no buffers bound
 };

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "engine.h"
+#include "batch.h"
 #include "post.h"
 #include "tswrtm.h"
 
@@ -390,6 +391,7 @@ int tsw_grid_set_sponge(tsw_grid_t h, const float* sponge) {
   TSW_HIP(hipMemcpyAsync(g->sponge.p, sponge, g->cells() * sizeof(float), hipMemcpyHostToDevice,
                          ctx->stream));
   TSW_HIP(hipStreamSynchronize(ctx->stream));
+  g->custom_sponge = true;
   return kOk;
 }
 
@@ -541,6 +543,45 @@ int tsw_migrate_shots(tsw_grid_t h, const float* vel_fwd, const float* vel_bwd, 
   TSW_TRY(shot_ill.ensure(nb));
   TSW_TRY(out.ensure(nb));
   TSW_TRY(maxbits.ensure(sizeof(unsigned)));
+  if (!g->custom_sponge && nshot >= 1) {
+    // All shots through the batch engine (temporally blocked steps, every
+    // shot resident at once as memory allows): each shot's image and
+    // illumination are bit-identical to migrate_shot_dev's, and they are
+    // stacked in shot order below, as the loop of the per-shot path does.
+    std::vector<tsw_shot_desc> d(nshot);
+    long long per = 0;
+    for (int i = 0; i < nshot; ++i) {
+      tsw_shot_desc& e = d[i];
+      std::memset(&e, 0, sizeof e);
+      e.nx = g->nx;
+      e.nz = g->nz;
+      e.nt = nts[i];
+      e.vel_fwd = vel_fwd;
+      e.vel_bwd = vel_bwd;
+      e.src = srcs[i];
+      e.six = sixs[i];
+      e.siz = sizs[i];
+      e.data = datas[i];
+      e.rix = rix;
+      e.nrec = nrec;
+      e.riz = riz;
+      e.kind = TSW_SHOT_ACOUSTIC;
+      per = std::max(per, ((long long)(nts[i] - 1) / k + 1 + 12) * n * (long long)sizeof(float));
+    }
+    size_t free_b = 0, total_b = 0;
+    TSW_HIP(hipMemGetInfo(&free_b, &total_b));
+    const long long fit = std::max<long long>(1, (long long)(0.7 * (double)free_b) / per);
+    const int slots = (int)std::min<long long>(std::min(nshot, 64), fit);
+    tsw_batch_t b = nullptr;
+    TSW_TRY(tsw_batch_create(nshot, d.data(), g->dx, g->dz, g->dt, g->n_sponge, g->alpha, g->fs, k,
+                             slots, 100, kDefaultBlockedKernel, &b));
+    int rc = tsw_batch_run(b, TSW_RUN_BATCHED);
+    if (rc == kOk) rc = batch_stack_device(b, g->nx, g->nz, stacked.f(), total.f(), st);
+    if (rc == kOk && hipStreamSynchronize(st) != hipSuccess) rc = kErrHip;
+    tsw_batch_destroy(b);
+    TSW_TRY(rc);
+    // each shot's illumination is already scaled by dt*k inside the batch
+  } else {
   launch_fill(stacked.f(), n, 0.0f, st);
   launch_fill(total.f(), n, 0.0f, st);
   for (int i = 0; i < nshot; ++i) {
@@ -552,6 +593,7 @@ int tsw_migrate_shots(tsw_grid_t h, const float* vel_fwd, const float* vel_bwd, 
     launch_stack_add(total.f(), shot_ill.f(), n, st);
     // the staging buffers g->src / g->data are overwritten by the next shot
     TSW_HIP(hipStreamSynchronize(st));
+  }
   }
   TSW_TRY(postprocess_dev(g, stacked.f(), total.f(), mute, out.f(), shot_img.f(),
                           static_cast<unsigned*>(maxbits.p), st));

@@ -224,3 +224,37 @@ def test_errors_and_injected_failure(engine):
     h = N.P()
     assert engine.tsw_grid_create(3, 16, 5.0, 5.0, 5e-4, 3, 0.0015, 1, ctypes.byref(h)) == -1
     assert "nx and nz" in N.last_error()
+
+
+def test_multiple_shots_batched_equals_per_shot_path():
+    """migrate_multiple_shots runs every shot through the batch engine when the
+    sponge is the default one, and shot by shot (per-step kernels, the
+    caller's per-cell sponge) once the caller replaces the sponge: with the
+    replacement equal to the default, both give the same bits."""
+    from das_amd.rtm_engine import TSWRTMEngine
+    from das_amd.velocity_flood import FloodModels
+    from das_amd.wave_solver import WaveSolver2D, ricker_wavelet
+    from das_amd.workloads import correlogram_gather, faulted_two_layer, smooth
+
+    nz, nx, dx, dt = 60, 150, 5.0, 4e-4
+    rng = np.random.default_rng(8)
+    vt = faulted_two_layer(nz, nx, dx, seed=2, depth_m=150.0, fault_x_m=300.0)
+    fm = FloodModels(vt, smooth(vt, 20.0, dx), smooth(vt, 40.0, dx), 0, 0)
+    rix = np.arange(0, nx, 3)
+    nts = [130, 97, 130, 61, 150]
+    srcs, datas, sixs = [], [], []
+    for nt in nts:
+        srcs.append(ricker_wavelet(25.0, dt, nt * dt)[0])
+        six = int(rng.integers(10, nx - 10))
+        sixs.append(six)
+        datas.append(correlogram_gather(rng, nt, dt, (rix - six) * dx, f0=25.0, f_lo=5, f_hi=50))
+    out = []
+    for force_per_shot in (False, True):
+        s = WaveSolver2D(nx, nz, dx, dx, dt, n_sponge=12)
+        if force_per_shot:  # same values, but now the caller's own sponge
+            N.check(N.lib().tsw_grid_set_sponge(s._grid, N.ptr(s.sponge)))
+        r = TSWRTMEngine(s).migrate_multiple_shots(fm, srcs, sixs, [2] * 5, datas, rix, 2, 2)
+        out.append(r)
+    for f in ("raw_image", "laplacian_image", "normalized_image"):
+        np.testing.assert_array_equal(getattr(out[0], f), getattr(out[1], f), err_msg=f)
+    assert np.abs(out[0].raw_image).max() > 0
