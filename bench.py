@@ -69,9 +69,9 @@ def elastic_leg(args, log):
 
     from das_amd.batch import ShotBatch
     from das_amd.wave_solver import WaveSolver2D
-    from das_amd.workloads import Config1, ElasticShot
+    from das_amd.workloads import PlainElasticShot, ElasticShot
 
-    cfg = Config1()
+    cfg = PlainElasticShot()
     solver = WaveSolver2D(cfg.nx, cfg.nz, cfg.dx, cfg.dx, cfg.dt, n_sponge=cfg.n_sponge,
                           solver_type="elastic")
     t0 = time.time()
@@ -109,9 +109,9 @@ def served_leg(args, acoustic_gcell, elastic_gcell, kernel, log):
     fraction of the capacity implied by the batched acoustic and elastic
     throughputs measured above."""
     from das_amd.serve import RTMServer, run_poisson
-    from das_amd.workloads import Config4
+    from das_amd.workloads import ServedStream
 
-    cfg = Config4(args.served_requests)
+    cfg = ServedStream(args.served_requests)
     t_req = np.mean([2 * s.nt * s.cells / 1e9 /
                      (elastic_gcell if s.kind == "elastic" else acoustic_gcell)
                      for s in cfg.specs])
@@ -153,9 +153,9 @@ def fwi_leg(args, log):
     import torch
 
     from das_amd.fwi import MisfitShot, WaveformMisfit
-    from das_amd.workloads import Config3
+    from das_amd.workloads import GradientShots
 
-    cfg = Config3()
+    cfg = GradientShots()
     src = cfg.source()
     geo = cfg.geometry()
 
@@ -326,7 +326,7 @@ def gather_leg(args, log):
 def traffic_from_profiles():
     """HBM bytes per launch of the step kernel from the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
-    p = os.path.join(ROOT, "profiles", "batch_step_config2_traffic.json")
+    p = os.path.join(ROOT, "profiles", "batch_step_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
@@ -363,7 +363,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU design "
-              "(shots sharded across GPUs, one image reduce) is described in DESIGN.md",
+              "(shots sharded across GPUs, one image reduce) is not part of this build",
               file=sys.stderr)
         sys.exit(2)
 
@@ -371,11 +371,11 @@ def main():
 
     from das_amd import _native as N
     from das_amd.batch import ShotBatch
-    from das_amd.workloads import Config2
+    from das_amd.workloads import BatchedWindows
 
     N.check(N.lib().tsw_init(0))
     t0 = time.time()
-    cfg = Config2(n_shots=args.shots).build(with_data=True)
+    cfg = BatchedWindows(n_shots=args.shots).build(with_data=True)
     log(f"workload built in {time.time() - t0:.1f}s")
     tuning = None
     if args.kernel < 0 or args.slots < 0:

@@ -92,7 +92,7 @@ class ElasticShot:
 
 
 @dataclass
-class Config0:
+class PlainAcousticShot:
     """Plain acoustic RTM, one virtual shot: 200 x 400 at 5 m, dt 0.5 ms, nt 4000."""
     nz: int = 200
     nx: int = 400
@@ -118,7 +118,7 @@ class Config0:
         self.rix = np.arange(200, dtype=np.int32) * 2
 
 
-def make_data_config0(cfg: Config0, solver):
+def make_plain_acoustic_data(cfg: PlainAcousticShot, solver):
     """Receiver data: forward-modelled in the true model, band-passed 0.8-4.5 Hz."""
     rec, _ = solver.forward_propagate_acoustic(cfg.true_vel, cfg.src, cfg.six, cfg.siz, cfg.rix,
                                                cfg.riz)
@@ -153,7 +153,7 @@ def elastic_from_vp(vp):
 
 
 @dataclass
-class Config1:
+class PlainElasticShot:
     """Plain 2D elastic P-SV RTM, one shot: 400 x 1600 at 5 m, dt 0.4 ms,
     nt 8000; vertical-force Ricker 2 Hz; 800 vertical strain-rate receivers
     at 10 m spacing."""
@@ -216,7 +216,7 @@ def correlogram_gather(rng, nt, dt, offsets_m, v_app=1800.0, f0=2.5, f_lo=0.8, f
 
 
 @dataclass
-class Config2:
+class BatchedWindows:
     """Batched acoustic RTM: 64 virtual shots on cropped windows nz = 300,
     nx ~ U{600..1600}, nt ~ U{3000..6000}, dx 5 m, dt 0.5 ms, k 2."""
     n_shots: int = 64
@@ -262,7 +262,7 @@ class Config2:
 
 
 @dataclass
-class Config3:
+class GradientShots:
     """Misfit gradient: 300 x 800 at 5 m, dt 0.5 ms, nt 5000, 16 surface shots
     (Ricker 2.5 Hz) over 400 receivers every 10 m; observed data modelled in
     the two-layer faulted model, gradient taken at its 100 m smoothing."""
@@ -311,7 +311,7 @@ class RequestSpec:
         return self.nx * self.nz
 
 
-class Config4:
+class ServedStream:
     """Served RTM: a seeded stream of virtual-shot requests, nz in
     {256, 384, 512} x nx in {512, 1024, 2048}, nt ~ U{2000..8000}, 80% acoustic
     and 20% elastic, all cropped from one shared faulted model, each with its

@@ -1,6 +1,6 @@
 """Elastic path on the GPU: the reference's elastic step (golden vectors),
 the elastic driver loops (golden vectors made with the reference kernel) and
-the config-1-shaped migration against the oracle (FP32 bit-exact, FP64 error
+the migration of an elastic shot (200 x 400, nt 1200) against the oracle (FP32 bit-exact, FP64 error
 stated)."""
 import numpy as np
 import pytest
@@ -44,14 +44,14 @@ def test_elastic_loops_golden(golden):
 
 
 def test_elastic_migrate_vs_oracle():
-    """A config-1-shaped shot, reduced to run the CPU oracle quickly
+    """An elastic shot of the 400 x 1600 workload's kind, reduced to run the CPU oracle quickly
     (200 x 400, nt 1200): FP32 bit-exact to the oracle, FP64 error stated."""
     from das_amd.rtm_engine import TSWRTMEngine
     from das_amd.wave_solver import WaveSolver2D
-    from das_amd.workloads import Config1
+    from das_amd.workloads import PlainElasticShot
     from oracle import oracle as ora
 
-    cfg = Config1(nz=200, nx=400, nt=1200, n_rec=200)
+    cfg = PlainElasticShot(nz=200, nx=400, nt=1200, n_rec=200)
     s = WaveSolver2D(cfg.nx, cfg.nz, cfg.dx, cfg.dx, cfg.dt, n_sponge=cfg.n_sponge,
                      solver_type="elastic")
     rec, _ = s.forward_propagate_elastic(cfg.model_true, cfg.src, cfg.six, cfg.siz, cfg.rix,

@@ -146,16 +146,16 @@ def test_rtm_engine_golden(golden):
                                       g[f"lap{i}_out"])
 
 
-def test_config0_full_size_vs_oracle():
+def test_plain_acoustic_200x400_full_size_vs_oracle():
     """Config 0 (200 x 400, nt 4000, k 2) end to end against the FP32 oracle;
     error against the FP64 oracle stated."""
     from das_amd.rtm_engine import TSWRTMEngine
     from das_amd.velocity_flood import flood_models_from
     from das_amd.wave_solver import WaveSolver2D
-    from das_amd.workloads import Config0, bandpass
+    from das_amd.workloads import PlainAcousticShot, bandpass
     from oracle import oracle as ora
 
-    cfg = Config0()
+    cfg = PlainAcousticShot()
     s = WaveSolver2D(cfg.nx, cfg.nz, cfg.dx, cfg.dx, cfg.dt, n_sponge=cfg.n_sponge)
     rec, _ = s.forward_propagate_acoustic(cfg.true_vel, cfg.src, cfg.six, cfg.siz, cfg.rix, cfg.riz)
     g = ora.Grid(cfg.nx, cfg.nz, cfg.dx, cfg.dx, cfg.dt, cfg.n_sponge)
@@ -169,9 +169,9 @@ def test_config0_full_size_vs_oracle():
     img_d, ill_d = ora.migrate_shot(g, cfg.mig_vel, cfg.mig_vel, cfg.src, cfg.six, cfg.siz, data,
                                     cfg.rix, cfg.riz, cfg.k, dtype=np.float64)
     e_img, e_ill = rel_l2(img, img_o), rel_l2(ill, ill_o)
-    print(f"\nconfig0 rel L2 vs FP32 oracle: rec {e_rec:.2e} image {e_img:.2e} illum {e_ill:.2e}"
+    print(f"\n200x400 shot rel L2 vs FP32 oracle: rec {e_rec:.2e} image {e_img:.2e} illum {e_ill:.2e}"
           f" (bit-exact image: {np.array_equal(img, img_o)})")
-    print(f"config0 rel L2 vs FP64 oracle: image GPU {rel_l2(img, img_d):.2e} "
+    print(f"200x400 shot rel L2 vs FP64 oracle: image GPU {rel_l2(img, img_d):.2e} "
           f"CPU-FP32 {rel_l2(img_o, img_d):.2e}; illum GPU {rel_l2(ill, ill_d):.2e}")
     assert e_rec <= REL_TOL and e_img <= REL_TOL and e_ill <= REL_TOL
     assert np.abs(img).max() > 0
