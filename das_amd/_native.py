@@ -74,6 +74,10 @@ SIGNATURES = {
     "tsw_fwi_destroy": (I, [P]),
     "tsw_xcorr": (I, [P, I, I, P, I, P, I, I, I, P]),
     "tsw_flood_columns": (I, [P, I, I, I, P, I, D, P]),
+    "tsw_sosfiltfilt": (I, [P, I, I, P, I, P, I, I, I, P]),
+    "tsw_sign": (I, [P, LL, P]),
+    "tsw_spectral_whitening": (I, [P, I, I, D, D, D, D, P]),
+    "tsw_fk_filter": (I, [P, I, I, D, D, D, D, P]),
     "tsw_save_matrix_csv": (I, [ctypes.c_char_p, P, I, I, I, P, P, ctypes.c_char_p,
                                 ctypes.c_char_p]),
     "tsw_xcorr_device": (I, [P, I, I, P, I, P, I, I, I, P, P]),

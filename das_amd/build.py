@@ -14,7 +14,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libtswrtm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["acoustic.hip", "elastic.hip", "post.hip", "engine.hip", "batch.hip", "elabatch.hip", "serve.hip", "fwi.hip", "vgather.hip", "models.hip", "output.hip"]
+SOURCES = ["acoustic.hip", "elastic.hip", "post.hip", "engine.hip", "batch.hip", "elabatch.hip", "serve.hip", "fwi.hip", "vgather.hip", "models.hip", "output.hip", "dsp.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -46,7 +46,7 @@ def build(verbose=False):
             if verbose and r.stderr.strip():
                 print(r.stderr, file=sys.stderr)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lhipfft"])
     os.replace(tmp, LIB)
     return LIB
 

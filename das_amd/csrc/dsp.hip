@@ -1,0 +1,318 @@
+// Preprocessing of continuous DAS data before correlation
+// (run_das_pipeline.py:165-168, :280).  The reference's preprocessing module
+// is absent from its source tree; the steps follow its README (README.md:27-52)
+// and the scipy routines named there:
+//   - zero-phase IIR filtering, scipy.signal.sosfiltfilt semantics (odd
+//     extension of padlen samples, steady-state initial conditions scaled by
+//     the first sample, forward pass, reversed pass), in FP64 with scipy's
+//     direct-form-II-transposed recurrence, optionally keeping every q-th
+//     output sample (decimation) and/or its sign (1-bit normalisation);
+//   - spectral whitening X(f) / (|X(f)| + eps) inside [fmin, fmax], 0 outside;
+//   - an F-K fan filter keeping apparent velocities |f / k| in [vmin, vmax].
+// The FFTs are hipFFT library transforms (plain, batched).
+#include <hipfft/hipfft.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "engine.h"
+#include "tswrtm.h"
+
+namespace tsw {
+
+constexpr int kMaxSections = 16;
+
+struct SosArgs {
+  const float* x;   // (nt, nch) row-major
+  int nt, nch, nsec, padlen, step, sign;
+  double sos[kMaxSections][6];
+  double zi[kMaxSections][2];
+  double* tmp;      // (nt + 2 padlen, nch) forward pass
+  float* out;       // (ceil(nt / step), nch)
+};
+
+// extended sample e of channel c (odd extension, float32 as numpy forms it)
+__device__ __forceinline__ float sos_ext(const SosArgs& a, int c, int e) {
+  const int n = a.nt, p = a.padlen;
+  const float* x = a.x;
+  const long long s = a.nch;
+  if (e < p) return 2.0f * x[c] - x[(long long)(p - e) * s + c];
+  if (e < p + n) return x[(long long)(e - p) * s + c];
+  const int j = e - p - n;  // 0..p-1
+  return 2.0f * x[(long long)(n - 1) * s + c] - x[(long long)(n - 2 - j) * s + c];
+}
+
+// One thread per channel: consecutive channels are consecutive addresses.
+__global__ void k_sosfiltfilt(SosArgs a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.nch) return;
+  const int ne = a.nt + 2 * a.padlen;
+  double z[kMaxSections][2];
+  const double x0 = (double)sos_ext(a, c, 0);
+  for (int s = 0; s < a.nsec; ++s) {
+    z[s][0] = a.zi[s][0] * x0;
+    z[s][1] = a.zi[s][1] * x0;
+  }
+  double y = 0.0;
+  for (int e = 0; e < ne; ++e) {
+    double xn = (double)sos_ext(a, c, e);
+    for (int s = 0; s < a.nsec; ++s) {
+      const double xc = xn;
+      xn = a.sos[s][0] * xc + z[s][0];
+      z[s][0] = a.sos[s][1] * xc - a.sos[s][4] * xn + z[s][1];
+      z[s][1] = a.sos[s][2] * xc - a.sos[s][5] * xn;
+    }
+    y = xn;
+    a.tmp[(long long)e * a.nch + c] = y;
+  }
+  for (int s = 0; s < a.nsec; ++s) {
+    z[s][0] = a.zi[s][0] * y;
+    z[s][1] = a.zi[s][1] * y;
+  }
+  for (int e = ne - 1; e >= 0; --e) {
+    double xn = a.tmp[(long long)e * a.nch + c];
+    for (int s = 0; s < a.nsec; ++s) {
+      const double xc = xn;
+      xn = a.sos[s][0] * xc + z[s][0];
+      z[s][0] = a.sos[s][1] * xc - a.sos[s][4] * xn + z[s][1];
+      z[s][1] = a.sos[s][2] * xc - a.sos[s][5] * xn;
+    }
+    const int t = e - a.padlen;
+    if (t >= 0 && t < a.nt && t % a.step == 0) {
+      float v = (float)xn;
+      if (a.sign) v = v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f);
+      a.out[(long long)(t / a.step) * a.nch + c] = v;
+    }
+  }
+}
+
+// np.sign(x).astype(float32)
+__global__ void k_sign(const float* x, long long n, float* out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float v = x[i];
+    out[i] = v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : (v == v ? 0.0f : v));
+  }
+}
+
+// X(f) <- X(f) / ((|X(f)| + eps) nt) for bins in [k0, k1], 0 elsewhere (the
+// nt is the unnormalised inverse transform's).  X: (nf, nch), channel fastest.
+__global__ void k_whiten(hipfftComplex* X, int nf, int nch, int k0, int k1, float eps,
+                         float scale) {
+  const long long n = (long long)nf * nch;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int k = (int)(i / nch);
+    hipfftComplex v = X[i];
+    if (k < k0 || k > k1) {
+      v.x = 0.0f;
+      v.y = 0.0f;
+    } else {
+      const float d = (sqrtf(v.x * v.x + v.y * v.y) + eps) * scale;
+      v.x = v.x / d;
+      v.y = v.y / d;
+    }
+    X[i] = v;
+  }
+}
+
+// F-K fan: keep (f, k) with k != 0 and vmin <= |f| / |k| <= vmax; X (nt, nx)
+// row-major 2-D spectrum, frequencies fftfreq(nt, dt), wavenumbers fftfreq(nx, dx).
+__global__ void k_fk_mask(hipfftComplex* X, int nt, int nx, double dt, double dx, double vmin,
+                          double vmax, float scale) {
+  const long long n = (long long)nt * nx;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int it = (int)(i / nx), ix = (int)(i - (long long)it * nx);
+    const int ft = it <= (nt - 1) / 2 ? it : it - nt;
+    const int kx = ix <= (nx - 1) / 2 ? ix : ix - nx;
+    // numpy.fft.fftfreq: index * (1 / (n d))
+    const double f = fabs((double)ft * (1.0 / (nt * dt))), k = fabs((double)kx * (1.0 / (nx * dx)));
+    bool keep = false;
+    if (kx != 0) {
+      const double v = f / k;
+      keep = v >= vmin && v <= vmax;
+    }
+    hipfftComplex c = X[i];
+    if (keep) {
+      c.x *= scale;
+      c.y *= scale;
+    } else {
+      c.x = 0.0f;
+      c.y = 0.0f;
+    }
+    X[i] = c;
+  }
+}
+
+__global__ void k_real_part(const hipfftComplex* X, long long n, float* out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = X[i].x;
+}
+
+__global__ void k_to_complex(const float* x, long long n, hipfftComplex* out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    hipfftComplex c;
+    c.x = x[i];
+    c.y = 0.0f;
+    out[i] = c;
+  }
+}
+
+static unsigned gridn(long long n) {
+  return (unsigned)std::max<long long>(1, std::min<long long>((n + 255) / 256, 65536));
+}
+
+#define TSW_FFT(call)                                                              \
+  do {                                                                             \
+    hipfftResult r_ = (call);                                                      \
+    if (r_ != HIPFFT_SUCCESS) {                                                    \
+      ::tsw::set_error("%s failed: hipfft status %d", #call, (int)r_);             \
+      return ::tsw::kErrHip;                                                       \
+    }                                                                              \
+  } while (0)
+
+struct FftPlan {
+  hipfftHandle h = 0;
+  bool made = false;
+  ~FftPlan() {
+    if (made) (void)hipfftDestroy(h);
+  }
+};
+
+}  // namespace tsw
+
+using namespace tsw;
+
+extern "C" {
+
+int tsw_sosfiltfilt(const float* x, int nt, int nch, const double* sos, int nsec,
+                    const double* zi, int padlen, int step, int sign, float* out) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  TSW_ARG(x && sos && zi && out, "null argument");
+  TSW_ARG(nt >= 1 && nch >= 1 && step >= 1, "bad sizes");
+  TSW_ARG(nsec >= 1 && nsec <= kMaxSections, "1 to %d second-order sections", kMaxSections);
+  TSW_ARG(padlen >= 0 && padlen < nt,
+          "the input must be longer than padlen (%d samples, as sosfiltfilt requires)", padlen);
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  hipStream_t st = ctx->stream;
+  SosArgs a;
+  std::memset(&a, 0, sizeof a);
+  for (int s = 0; s < nsec; ++s) {
+    TSW_ARG(sos[s * 6 + 3] == 1.0, "section %d is not normalised (a0 = %g)", s, sos[s * 6 + 3]);
+    for (int j = 0; j < 6; ++j) a.sos[s][j] = sos[s * 6 + j];
+    a.zi[s][0] = zi[s * 2];
+    a.zi[s][1] = zi[s * 2 + 1];
+  }
+  a.nt = nt;
+  a.nch = nch;
+  a.nsec = nsec;
+  a.padlen = padlen;
+  a.step = step;
+  a.sign = sign ? 1 : 0;
+  const int nout = (nt + step - 1) / step;
+  DevBuf dx, dtmp, dout;
+  TSW_TRY(dx.ensure((size_t)nt * nch * sizeof(float)));
+  TSW_TRY(dtmp.ensure((size_t)(nt + 2 * padlen) * nch * sizeof(double)));
+  TSW_TRY(dout.ensure((size_t)nout * nch * sizeof(float)));
+  TSW_HIP(hipMemcpyAsync(dx.p, x, (size_t)nt * nch * sizeof(float), hipMemcpyHostToDevice, st));
+  a.x = dx.f();
+  a.tmp = (double*)dtmp.p;
+  a.out = dout.f();
+  hipLaunchKernelGGL(k_sosfiltfilt, dim3((nch + 63) / 64), dim3(64), 0, st, a);
+  TSW_HIP(hipGetLastError());
+  TSW_HIP(hipMemcpyAsync(out, dout.p, (size_t)nout * nch * sizeof(float), hipMemcpyDeviceToHost,
+                         st));
+  TSW_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+int tsw_sign(const float* x, long long n, float* out) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  TSW_ARG(x && out && n >= 0, "bad arguments");
+  if (n == 0) return kOk;
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  hipStream_t st = ctx->stream;
+  DevBuf a, b;
+  TSW_TRY(a.ensure((size_t)n * sizeof(float)));
+  TSW_TRY(b.ensure((size_t)n * sizeof(float)));
+  TSW_HIP(hipMemcpyAsync(a.p, x, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_sign, dim3(gridn(n)), dim3(256), 0, st, a.f(), n, b.f());
+  TSW_HIP(hipGetLastError());
+  TSW_HIP(hipMemcpyAsync(out, b.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st));
+  TSW_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+int tsw_spectral_whitening(const float* x, int nt, int nch, double fs, double fmin, double fmax,
+                           double eps, float* out) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  TSW_ARG(x && out && nt >= 2 && nch >= 1 && fs > 0 && fmin <= fmax && eps >= 0,
+          "bad arguments");
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  hipStream_t st = ctx->stream;
+  const int nf = nt / 2 + 1;
+  // bins k with fmin <= k fs / nt <= fmax
+  const int k0 = std::max(0, (int)std::ceil(fmin * nt / fs - 1e-9));
+  const int k1 = std::min(nf - 1, (int)std::floor(fmax * nt / fs + 1e-9));
+  DevBuf dx, dX;
+  TSW_TRY(dx.ensure((size_t)nt * nch * sizeof(float)));
+  TSW_TRY(dX.ensure((size_t)nf * nch * sizeof(hipfftComplex)));
+  TSW_HIP(hipMemcpyAsync(dx.p, x, (size_t)nt * nch * sizeof(float), hipMemcpyHostToDevice, st));
+  int n[1] = {nt};
+  int inembed[1] = {nt}, onembed[1] = {nf};
+  FftPlan fwd, inv;
+  TSW_FFT(hipfftPlanMany(&fwd.h, 1, n, inembed, nch, 1, onembed, nch, 1, HIPFFT_R2C, nch));
+  fwd.made = true;
+  TSW_FFT(hipfftPlanMany(&inv.h, 1, n, onembed, nch, 1, inembed, nch, 1, HIPFFT_C2R, nch));
+  inv.made = true;
+  TSW_FFT(hipfftSetStream(fwd.h, st));
+  TSW_FFT(hipfftSetStream(inv.h, st));
+  TSW_FFT(hipfftExecR2C(fwd.h, dx.f(), (hipfftComplex*)dX.p));
+  hipLaunchKernelGGL(k_whiten, dim3(gridn((long long)nf * nch)), dim3(256), 0, st,
+                     (hipfftComplex*)dX.p, nf, nch, k0, k1, (float)eps, (float)nt);
+  TSW_HIP(hipGetLastError());
+  TSW_FFT(hipfftExecC2R(inv.h, (hipfftComplex*)dX.p, dx.f()));
+  TSW_HIP(hipMemcpyAsync(out, dx.p, (size_t)nt * nch * sizeof(float), hipMemcpyDeviceToHost, st));
+  TSW_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+int tsw_fk_filter(const float* g, int nt, int nx, double dt, double dx, double vmin, double vmax,
+                  float* out) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  TSW_ARG(g && out && nt >= 1 && nx >= 1 && dt > 0 && dx > 0 && vmin <= vmax, "bad arguments");
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  hipStream_t st = ctx->stream;
+  const long long n = (long long)nt * nx;
+  DevBuf dg, dX;
+  TSW_TRY(dg.ensure((size_t)n * sizeof(float)));
+  TSW_TRY(dX.ensure((size_t)n * sizeof(hipfftComplex)));
+  TSW_HIP(hipMemcpyAsync(dg.p, g, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_to_complex, dim3(gridn(n)), dim3(256), 0, st, dg.f(), n,
+                     (hipfftComplex*)dX.p);
+  FftPlan p;
+  TSW_FFT(hipfftPlan2d(&p.h, nt, nx, HIPFFT_C2C));
+  p.made = true;
+  TSW_FFT(hipfftSetStream(p.h, st));
+  TSW_FFT(hipfftExecC2C(p.h, (hipfftComplex*)dX.p, (hipfftComplex*)dX.p, HIPFFT_FORWARD));
+  hipLaunchKernelGGL(k_fk_mask, dim3(gridn(n)), dim3(256), 0, st, (hipfftComplex*)dX.p, nt, nx,
+                     dt, dx, vmin, vmax, 1.0f / (float)n);
+  TSW_HIP(hipGetLastError());
+  TSW_FFT(hipfftExecC2C(p.h, (hipfftComplex*)dX.p, (hipfftComplex*)dX.p, HIPFFT_BACKWARD));
+  hipLaunchKernelGGL(k_real_part, dim3(gridn(n)), dim3(256), 0, st, (const hipfftComplex*)dX.p,
+                     n, dg.f());
+  TSW_HIP(hipGetLastError());
+  TSW_HIP(hipMemcpyAsync(out, dg.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st));
+  TSW_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+}  // extern "C"

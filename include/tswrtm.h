@@ -323,6 +323,27 @@ int tsw_save_matrix_csv(const char* path, const void* matrix, int precision, int
                         const double* row_axis, const double* col_axis, const char* row_name,
                         const char* col_name);
 
+/* Preprocessing of continuous data (run_das_pipeline.py:165-168, :280; the
+ * reference's preprocessing module is absent, the steps follow README.md:27-52).
+ *
+ * Zero-phase filtering of x (nt, nch) along time with scipy.signal.sosfiltfilt
+ * semantics: `sos` (nsec, 6) normalised sections, `zi` (nsec, 2) =
+ * sosfilt_zi(sos), odd extension of `padlen` samples; FP64 arithmetic in
+ * scipy's order.  Every `step`-th output sample is kept (decimation; out is
+ * (ceil(nt/step), nch)), as its sign when sign != 0 (1-bit). */
+int tsw_sosfiltfilt(const float* x, int nt, int nch, const double* sos, int nsec,
+                    const double* zi, int padlen, int step, int sign, float* out);
+/* np.sign(x).astype(float32) of n values. */
+int tsw_sign(const float* x, long long n, float* out);
+/* Spectral whitening along time of x (nt, nch): rfft, X / (|X| + eps) for
+ * fmin <= f <= fmax and 0 elsewhere, irfft (FP32 transforms). */
+int tsw_spectral_whitening(const float* x, int nt, int nch, double fs, double fmin, double fmax,
+                           double eps, float* out);
+/* F-K fan filter of a gather g (nt, nx): 2-D FFT, keep wavenumbers k != 0 with
+ * vmin <= |f| / |k| <= vmax, inverse FFT, real part (FP32 transforms). */
+int tsw_fk_filter(const float* g, int nt, int nx, double dt, double dx, double vmin, double vmax,
+                  float* out);
+
 #ifdef __cplusplus
 }
 #endif
