@@ -169,3 +169,29 @@ def test_mixed_acoustic_elastic_batch(mixed):
         info = b.info()
         assert info["elastic_cell_updates"] == sum(2 * s.nt * s.nx * s.nz for s in eshots)
         b.close()
+
+
+def test_full_size_windows_batched_equal_single_shots():
+    """Four windows of the 64-shot workload at full size (nz 300, nx
+    600-1600, nt 3000-6000): the tuned blocked kernel with every shot resident
+    equals migrating each alone, bit for bit."""
+    from das_amd.batch import ShotBatch
+    from das_amd.rtm_engine import TSWRTMEngine
+    from das_amd.velocity_flood import FloodModels
+    from das_amd.wave_solver import WaveSolver2D
+    from das_amd.workloads import BatchedWindows
+
+    cfg = BatchedWindows(n_shots=4).build(with_data=True)
+    b = ShotBatch(cfg.shots, cfg.dx, cfg.dx, cfg.dt, n_sponge=cfg.n_sponge, k=cfg.k, slots=4,
+                  kernel=15, graph_steps=100)
+    b.run("batched")
+    for i, s in enumerate(cfg.shots):
+        solver = WaveSolver2D(s.nx, s.nz, cfg.dx, cfg.dx, cfg.dt, n_sponge=cfg.n_sponge)
+        img, ill = TSWRTMEngine(solver).migrate_shot(
+            FloodModels(s.vel_fwd, s.vel_fwd, s.vel_bwd, 0, 0), s.src, s.six, s.siz, s.data,
+            s.rix, s.riz, cfg.k)
+        bi, bl = b.shot_result(i)
+        print(f"shot {i}: {s.nz}x{s.nx}, nt={s.nt}", flush=True)
+        np.testing.assert_array_equal(bi, img)
+        np.testing.assert_array_equal(bl, ill)
+    b.close()

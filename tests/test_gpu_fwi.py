@@ -290,3 +290,32 @@ def test_blocked_forward_equals_per_step(config):
         ref.close()
     with pytest.raises(ValueError):
         _op(shots[:1], np.float64).set_blocking(1)
+
+
+def test_full_size_checkpointed_equals_stored():
+    """The 300 x 800, nt 5000 gradient workload (4 of its 16 shots): sqrt(nt)
+    checkpoints with recompute give the gradient of keeping every level, bit
+    for bit, in FP32 with the blocked forward."""
+    from das_amd.fwi import MisfitShot, WaveformMisfit
+    from das_amd.workloads import GradientShots
+
+    cfg = GradientShots()
+    src = cfg.source()
+    geo = cfg.geometry()[::4]
+    shots = [MisfitShot(src=src, six=a, siz=b, rix=r, riz=z,
+                        d_obs=np.zeros((cfg.nt, len(r)), np.float32)) for a, b, r, z in geo]
+    op = WaveformMisfit(cfg.nx, cfg.nz, cfg.dx, cfg.dx, cfg.dt, shots, n_sponge=cfg.n_sponge)
+    op.forward_loss(_dev(cfg.true_vel, np.float32))
+    obs = [op.predicted(i) for i in range(len(shots))]
+    op.close()
+    for s, d in zip(shots, obs):
+        s.d_obs = d
+    v = _dev(cfg.start_vel, np.float32)
+    grads = []
+    for ck in (0, -1):
+        op = WaveformMisfit(cfg.nx, cfg.nz, cfg.dx, cfg.dx, cfg.dt, shots,
+                            n_sponge=cfg.n_sponge, ckpt_interval=ck)
+        grads.append(op.gradient(v).cpu().numpy())
+        op.close()
+    assert np.abs(grads[0]).max() > 0
+    np.testing.assert_array_equal(grads[0], grads[1])
