@@ -196,6 +196,10 @@ struct AdjArgs {
   const FwiMeta* meta;
   int res_it;     // residual row added to Lam^n (n-1), -1: none
   int probe_it;
+  // two-step kernel: Lam^{n-1} too
+  T* lamm1;       // Lam^{n-1} (written)
+  const T* um1;   // forward u^{n-1}
+  int res_it2;    // residual row added to Lam^{n-1} (n-2), -1: none
 };
 
 template <typename T>
@@ -274,6 +278,126 @@ __global__ __launch_bounds__(256) void k_fwi_adj(AdjArgs<T> a) {
       for (int r = r0; r < r1; ++r) acc = acc + row[m.ridx[r]];
     }
     a.lam0[o + c] = acc;
+  }
+}
+
+// Two adjoint steps in one launch: Lam^n on the tile and a 2-cell halo, then
+// Lam^{n-1} on the tile.  The step formulas, their FP order and the order of
+// the two gradient additions are those of k_fwi_adj applied at n then n-1, so
+// the result is bit-identical to two single steps.  Reads Lam^{n+1} (tile +
+// 4), Lam^{n+2}, Lam^{n+3} (border cells), u^n, u^{n-1}; writes Lam^n,
+// Lam^{n-1} into buffers no workgroup of the launch reads (five rotate).
+template <typename T, int TZ>
+__global__ __launch_bounds__(256) void k_fwi_adj2(AdjArgs<T> a) {
+  constexpr int W1 = 64 + 4, H1 = TZ + 4;  // tile + 2
+  constexpr int W2 = 64 + 8, H2 = TZ + 8;  // tile + 4
+  __shared__ T cw1[H2 * W2];               // C.w^{n+1}
+  __shared__ T ln[H1 * W1];                // Lam^n
+  __shared__ T cw0[H1 * W1];               // C.w^n
+  const int nx = a.nx, nz = a.nz;
+  const int x0 = blockIdx.x * 64, z0 = blockIdx.y * TZ;
+  const long long o = (long long)blockIdx.z * a.n;
+  const T* lam1 = a.lam1 + o;
+  const T* lam2 = a.lam2 + o;
+  const FwiMeta& m = a.meta[blockIdx.z];
+  const T c0 = FwiK<T>::c0, c1 = fwi_c1<T>(), c2 = fwi_c2<T>();
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  for (int i = tid; i < H2 * W2; i += 256) {
+    const int gz = z0 - 4 + i / W2, gx = x0 - 4 + i % W2;
+    T v = T(0);
+    if (fwi_interior<T>(gz, gx, nz, nx)) {
+      const int c = gz * nx + gx;
+      v = a.C[c] * (fwi_sponge(a, gz, gx, c) * fwi_zeta(a.fs, lam1, c, gz, nx));
+    }
+    cw1[i] = v;
+  }
+  __syncthreads();
+  // step n on tile + 2
+  for (int i = tid; i < H1 * W1; i += 256) {
+    const int lzz = i / W1, lxx = i % W1;
+    const int gz = z0 - 2 + lzz, gx = x0 - 2 + lxx;
+    if (gz < 0 || gz >= nz || gx < 0 || gx >= nx) {
+      ln[i] = T(0);
+      continue;
+    }
+    const int c = gz * nx + gx;
+    const int l = (lzz + 2) * W2 + lxx + 2;
+    T lx = c2 * (cw1[l - 2] + cw1[l + 2]);
+    lx = lx + c1 * (cw1[l - 1] + cw1[l + 1]);
+    lx = (lx + c0 * cw1[l]) * a.rdx2;
+    T lz2 = c2 * (cw1[l - 2 * W2] + cw1[l + 2 * W2]);
+    lz2 = lz2 + c1 * (cw1[l - W2] + cw1[l + W2]);
+    lz2 = (lz2 + c0 * cw1[l]) * a.rdz2;
+    T acc;
+    if (fwi_interior<T>(gz, gx, nz, nx)) {
+      const T sc = fwi_sponge(a, gz, gx, c);
+      const T w1 = sc * fwi_zeta(a.fs, lam1, c, gz, nx);
+      const T w2 = sc * fwi_zeta(a.fs, lam2, c, gz, nx);
+      acc = (T(2) * w1 + (lx + lz2)) - w2;
+    } else {
+      acc = lx + lz2;
+      if (!(a.fs && gz < 2)) acc = acc + fwi_zeta(a.fs, a.lam3 + o, c, gz, nx);
+    }
+    const bool core = lzz >= 2 && lzz < 2 + TZ && lxx >= 2 && lxx < 2 + 64;
+    if (core && c == m.src_cell) ((T*)m.probe)[a.probe_it] = acc;
+    if (a.res_it >= 0 && gz == m.riz) {
+      const int r0 = m.roff[gx], r1 = m.roff[gx + 1];
+      const T* row = (const T*)m.res + (long long)a.res_it * m.nrec;
+      for (int r = r0; r < r1; ++r) acc = acc + row[m.ridx[r]];
+    }
+    ln[i] = acc;
+    if (core) a.lam0[o + c] = acc;
+  }
+  __syncthreads();
+  // C.w^n on tile + 2 from Lam^n (Q^T at row 2 reads row 1 of the same region)
+  for (int i = tid; i < H1 * W1; i += 256) {
+    const int gz = z0 - 2 + i / W1, gx = x0 - 2 + i % W1;
+    T v = T(0);
+    if (fwi_interior<T>(gz, gx, nz, nx)) {
+      const int c = gz * nx + gx;
+      T zt = ln[i];
+      if (a.fs && gz == 2) zt = ln[i] - ln[i - W1];
+      v = a.C[c] * (fwi_sponge(a, gz, gx, c) * zt);
+    }
+    cw0[i] = v;
+  }
+  __syncthreads();
+  const int ix = x0 + threadIdx.x;
+  if (ix >= nx) return;
+  for (int k = 0; k < TZ / 4; ++k) {
+    const int lzz = threadIdx.y + 4 * k, iz = z0 + lzz;
+    if (iz >= nz) break;
+    const int c = iz * nx + ix;
+    const int l = (lzz + 2) * W1 + threadIdx.x + 2;
+    T lx = c2 * (cw0[l - 2] + cw0[l + 2]);
+    lx = lx + c1 * (cw0[l - 1] + cw0[l + 1]);
+    lx = (lx + c0 * cw0[l]) * a.rdx2;
+    T lz2 = c2 * (cw0[l - 2 * W1] + cw0[l + 2 * W1]);
+    lz2 = lz2 + c1 * (cw0[l - W1] + cw0[l + W1]);
+    lz2 = (lz2 + c0 * cw0[l]) * a.rdz2;
+    T acc;
+    if (fwi_interior<T>(iz, ix, nz, nx)) {
+      const T sc = fwi_sponge(a, iz, ix, c);
+      T zt = ln[l];
+      if (a.fs && iz == 2) zt = ln[l] - ln[l - W1];
+      const T w0 = sc * zt;
+      const T wp = sc * fwi_zeta(a.fs, lam1, c, iz, nx);
+      acc = (T(2) * w0 + (lx + lz2)) - wp;
+      T g = a.grad[o + c];
+      g = g + wp * lap4(a.un + o, c, nx, a.rdx2, a.rdz2);
+      g = g + w0 * lap4(a.um1 + o, c, nx, a.rdx2, a.rdz2);
+      a.grad[o + c] = g;
+    } else {
+      acc = lx + lz2;
+      if (!(a.fs && iz < 2)) acc = acc + fwi_zeta(a.fs, lam2, c, iz, nx);
+    }
+    if (c == m.src_cell) ((T*)m.probe)[a.probe_it - 1] = acc;
+    if (a.res_it2 >= 0 && iz == m.riz) {
+      const int r0 = m.roff[ix], r1 = m.roff[ix + 1];
+      const T* row = (const T*)m.res + (long long)a.res_it2 * m.nrec;
+      for (int r = r0; r < r1; ++r) acc = acc + row[m.ridx[r]];
+    }
+    a.lamm1[o + c] = acc;
   }
 }
 
@@ -549,6 +673,7 @@ struct Fwi {
   std::vector<FwiGroup> groups;
   DevBuf S, C, buf, mu, seg, gC, partial, loss;
   int tb = 0;          // FP32 blocked forward: configuration kFwiTb[tb - 1]; 0 = per step
+  int adj_steps = 1;   // adjoint steps per launch (1 or 2; 2 measured slower on MI355X)
   DevBuf sets, table;  // blocked forward: two state sets [2][3][cmax][n], sponge table
   double last_loss = 0;
   bool have_forward = false;
@@ -584,9 +709,11 @@ template <typename T>
 static T* lvl(const Fwi* F, int b) {
   return (T*)F->buf.p + (long long)b * F->cmax * F->cells();
 }
+constexpr int kAdjRing = 5;  // adjoint levels: Lam^{n-1} .. Lam^{n+3}
+
 template <typename T>
 static T* adj(const Fwi* F, int b) {
-  return (T*)F->mu.p + (long long)b * F->cmax * F->cells();
+  return (T*)F->mu.p + (long long)(b % kAdjRing) * F->cmax * F->cells();
 }
 
 template <typename T>
@@ -749,11 +876,11 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
   for (auto& G : F->groups) {
     const int nt = G.nt, cnt = G.count();
     const long long cn = (long long)cnt * n;
-    for (int b = 0; b < 4; ++b)
+    for (int b = 0; b < kAdjRing; ++b)
       hipLaunchKernelGGL(k_fwi_fill<T>, dim3(g1(cn)), dim3(256), 0, st, adj<T>(F, b), cn, T(0));
-    // Lam^{nt} = R^T r_{nt-1} lives in buffer nt % 4; Lam^{nt+1..nt+3} = 0
+    // Lam^{nt} = R^T r_{nt-1}; Lam^{nt+1..nt+3} = 0
     hipLaunchKernelGGL(k_fwi_seed<T>, dim3((F->nx + 255) / 256, cnt), dim3(256), 0, st,
-                       adj<T>(F, nt % 4), n, F->nx, (const FwiMeta*)G.meta.p, nt - 1);
+                       adj<T>(F, nt), n, F->nx, (const FwiMeta*)G.meta.p, nt - 1);
     AdjArgs<T> a;
     std::memset(&a, 0, sizeof a);
     const T dxT = (T)F->dx, dzT = (T)F->dz;
@@ -803,16 +930,26 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
       for (int it = n0; it < n1; ++it)
         fwd_step<T>(F, G, s, it, (T*)F->seg.p + (long long)(it - n0) * cn, false, st);
       }
-      for (int it = n1 - 1; it >= n0; --it) {
-        a.lam0 = adj<T>(F, it % 4);
-        a.lam1 = adj<T>(F, (it + 1) % 4);
-        a.lam2 = adj<T>(F, (it + 2) % 4);
-        a.lam3 = adj<T>(F, (it + 3) % 4);
+      for (int it = n1 - 1; it >= n0;) {
+        a.lam0 = adj<T>(F, it);
+        a.lam1 = adj<T>(F, it + 1);
+        a.lam2 = adj<T>(F, it + 2);
+        a.lam3 = adj<T>(F, it + 3);
         a.un = (const T*)F->seg.p + (long long)(it - n0) * cn;
         a.res_it = it >= 1 ? it - 1 : -1;
         a.probe_it = it;
-        hipLaunchKernelGGL((k_fwi_adj<T, 16>), dim3((F->nx + 63) / 64, (F->nz + 15) / 16, cnt),
-                           dim3(64, 4), 0, st, a);
+        if (F->adj_steps == 2 && it - 1 >= n0) {
+          a.lamm1 = adj<T>(F, it + 4);  // Lam^{it-1}: the ring slot of Lam^{it+4}
+          a.um1 = (const T*)F->seg.p + (long long)(it - 1 - n0) * cn;
+          a.res_it2 = it >= 2 ? it - 2 : -1;
+          hipLaunchKernelGGL((k_fwi_adj2<T, 16>), dim3((F->nx + 63) / 64, (F->nz + 15) / 16, cnt),
+                             dim3(64, 4), 0, st, a);
+          it -= 2;
+        } else {
+          hipLaunchKernelGGL((k_fwi_adj<T, 16>), dim3((F->nx + 63) / 64, (F->nz + 15) / 16, cnt),
+                             dim3(64, 4), 0, st, a);
+          it -= 1;
+        }
       }
     }
     TSW_HIP(hipGetLastError());
@@ -948,7 +1085,7 @@ int tsw_fwi_create(int nx, int nz, double dx, double dz, double dt, int n_sponge
   }
   const size_t cn = (size_t)F->cmax * n * es;
   if ((rc = F->C.ensure(n * es)) || (rc = F->gC.ensure(cn)) || (rc = F->buf.ensure(3 * cn)) ||
-      (rc = F->mu.ensure(4 * cn)) ||
+      (rc = F->mu.ensure(kAdjRing * cn)) ||
       (rc = F->seg.ensure((size_t)(store_all ? ntmax : std::min(F->kint, ntmax)) * cn)) ||
       (rc = F->partial.ensure(kSumBlocks * sizeof(double))) ||
       (rc = F->loss.ensure(nshot * sizeof(double))))
@@ -999,6 +1136,13 @@ int tsw_fwi_set_blocking(tsw_fwi_t h, int config) {
   TSW_ARG(config == 0 || F->prec == 4, "the blocked forward is FP32 only");
   F->tb = config;
   F->have_forward = false;  // checkpoints of another path: run the forward again
+  return kOk;
+}
+
+int tsw_fwi_set_adjoint_steps(tsw_fwi_t h, int steps) {
+  Fwi* F = reinterpret_cast<Fwi*>(h);
+  TSW_ARG(F && (steps == 1 || steps == 2), "adjoint steps per launch must be 1 or 2");
+  F->adj_steps = steps;
   return kOk;
 }
 

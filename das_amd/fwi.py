@@ -132,6 +132,12 @@ class WaveformMisfit:
         N.check(N.lib().tsw_fwi_set_blocking(self._h, int(config)))
         self._last_vel = None
 
+    def set_adjoint_steps(self, steps: int):
+        """Adjoint steps per launch: 1 (default), or 2 (one launch computes two
+        consecutive levels; slower on MI355X at 300 x 800).  The gradient is
+        bit-identical either way."""
+        N.check(N.lib().tsw_fwi_set_adjoint_steps(self._h, int(steps)))
+
     def predicted(self, shot: int) -> np.ndarray:
         """Predicted data (nt, nrec) of a shot from the last forward."""
         nt, nrec = self.shots[shot]

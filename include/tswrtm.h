@@ -268,6 +268,8 @@ int tsw_fwi_backward(tsw_fwi_t fwi, const void* d_vel, void* d_grad, void* strea
 /* FP32 forward and recompute: 0 = one launch per step, c = 1..6 = temporally
  * blocked configuration c = 1..9 (the default is 3).  FP64 runs one launch per step. */
 int tsw_fwi_set_blocking(tsw_fwi_t fwi, int config);
+/* Adjoint steps per launch, 1 (default) or 2; the same bits either way. */
+int tsw_fwi_set_adjoint_steps(tsw_fwi_t fwi, int steps);
 /* Predicted data of a shot from the last forward, (nt, nrec), to the host. */
 int tsw_fwi_predicted(tsw_fwi_t fwi, int shot, void* host_out);
 /* Overwrites a shot's residual (nt, nrec) before tsw_fwi_backward (adjoint tests). */

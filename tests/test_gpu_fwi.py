@@ -319,3 +319,22 @@ def test_full_size_checkpointed_equals_stored():
         op.close()
     assert np.abs(grads[0]).max() > 0
     np.testing.assert_array_equal(grads[0], grads[1])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_two_step_adjoint_equals_single_steps(dtype):
+    """Two adjoint steps per launch give the single-step gradient and adjoint
+    field bit for bit, with segments of odd and even length."""
+    v_true, v0 = _models()
+    shots = _shots(v_true, n=2, dtype=dtype)
+    v = _dev(v0, dtype)
+    for ck in (7, 0, -1):
+        out = []
+        for steps in (1, 2):
+            op = _op(shots, dtype, ckpt=ck)
+            op.set_adjoint_steps(steps)
+            g = op.gradient(v).cpu().numpy()
+            out.append((g, op.adjoint_at_source(0), op.adjoint_at_source(1)))
+            op.close()
+        for a, b in zip(*out):
+            np.testing.assert_array_equal(a, b)
