@@ -1,0 +1,74 @@
+"""The virtual-source RTM chain around the migration path, on the GPU.
+
+Follows run_das_pipeline.py:160-370 step by step: per block of continuous
+data, decimation, band-pass, 1-bit and whitening; correlation of each
+virtual-source channel with the receiver channels, stacked over blocks and
+divided by the block count; per source the causal Green's function (time
+derivative) and an F-K fan filter; resampling to the simulation step; and
+multi-shot migration with Ricker sources.  Each step is the das_amd function
+of the same name as the reference's.
+"""
+from __future__ import annotations
+
+from typing import Iterable, List, Sequence, Tuple
+
+import numpy as np
+
+from .interferometry import build_virtual_shot_gather, cross_correlations, \
+    resample_gather_to_sim_dt
+from .preprocessing import bandpass_filter, decimate_time, fk_filter, spectral_whitening, \
+    temporal_normalization
+from .rtm_engine import RTMResult, TSWRTMEngine
+from .velocity_flood import FloodModels
+from .wave_solver import ricker_wavelet
+
+
+def stacked_correlations(blocks: Iterable[np.ndarray], fs_orig: float, target_fs: float,
+                         fmin: float, fmax: float, source_channels: Sequence[int],
+                         receiver_channels: np.ndarray, max_lag_s: float
+                         ) -> Tuple[np.ndarray, np.ndarray, float]:
+    """run_das_pipeline.py:160-213 and :232-235 -> (ccfs (n_src, 2 n_lags + 1,
+    n_rec) float32 averaged over the blocks, lags, dt_proc)."""
+    stacked, lags, dt_proc, n = None, None, None, 0
+    for raw in blocks:
+        dec, dt_proc, fs_proc = decimate_time(raw, fs_orig, target_fs)
+        bp = bandpass_filter(dec, fs_proc, fmin, fmax)
+        white = spectral_whitening(temporal_normalization(bp, "1bit"), fs_proc, fmin, fmax)
+        if stacked is None:
+            stacked, lags = cross_correlations(white, source_channels, max_lag_s, dt_proc,
+                                               receiver_channels)
+        else:
+            cross_correlations(white, source_channels, max_lag_s, dt_proc, receiver_channels,
+                               out=stacked)
+        n += 1
+    if stacked is None:
+        raise ValueError("no data blocks")
+    return stacked / np.float32(n) if n > 1 else stacked, lags, dt_proc
+
+
+def virtual_gathers(ccfs: np.ndarray, lags: np.ndarray, source_channels: Sequence[int],
+                    receiver_channels: np.ndarray, channel_coords_x: np.ndarray,
+                    receiver_spacing_m: float, fk_vmin: float, fk_vmax: float) -> List[np.ndarray]:
+    """run_das_pipeline.py:240-283: causal Green's function of each source's
+    stacked correlations, then the F-K fan filter."""
+    out = []
+    for v_idx, src in enumerate(source_channels):
+        vs = build_virtual_shot_gather(ccfs[v_idx], lags, v_idx, int(src), channel_coords_x,
+                                       receiver_channels, branch="causal", take_derivative=True)
+        out.append(fk_filter(vs.shot_gather, vs.dt, receiver_spacing_m, fk_vmin, fk_vmax))
+    return out
+
+
+def migrate_virtual_sources(engine: TSWRTMEngine, flood_models: FloodModels,
+                            gathers: Sequence[np.ndarray], dt_proc: float, max_lag_s: float,
+                            ricker_f0: float, source_ix: Sequence[int], receiver_ix: np.ndarray,
+                            snapshot_interval: int = 2) -> RTMResult:
+    """run_das_pipeline.py:337-370: gathers resampled to the solver's step,
+    Ricker sources of max_lag_s, multi-shot migration from row 2."""
+    dt_sim = engine.solver.dt
+    sim_nt = int(round(max_lag_s / dt_sim))
+    res = [resample_gather_to_sim_dt(g, dt_proc, dt_sim, sim_nt) for g in gathers]
+    src, _ = ricker_wavelet(ricker_f0, dt_sim, max_lag_s)
+    return engine.migrate_multiple_shots(flood_models, [src] * len(res), list(source_ix),
+                                         [2] * len(res), res, receiver_ix, 2, snapshot_interval,
+                                         mute_sponge=True)
