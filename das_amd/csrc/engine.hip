@@ -484,6 +484,32 @@ int tsw_migrate_shot(tsw_grid_t h, const float* vel_fwd, const float* vel_bwd, c
   TSW_TRY(ctx_get(&ctx));
   hipStream_t st = ctx->stream;
   const long long n = g->cells();
+  if (!g->custom_sponge) {
+    // one-shot batch: the temporally blocked kernel, bit-identical to the
+    // per-step path below
+    tsw_shot_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.nx = g->nx;
+    d.nz = g->nz;
+    d.nt = nt;
+    d.vel_fwd = vel_fwd;
+    d.vel_bwd = vel_bwd;
+    d.src = src;
+    d.six = six;
+    d.siz = siz;
+    d.data = data;
+    d.rix = rix;
+    d.nrec = nrec;
+    d.riz = riz;
+    d.kind = TSW_SHOT_ACOUSTIC;
+    tsw_batch_t b = nullptr;
+    TSW_TRY(tsw_batch_create(1, &d, g->dx, g->dz, g->dt, g->n_sponge, g->alpha, g->fs, k, 1, 0,
+                             kDefaultBlockedKernel, &b));
+    int rc = tsw_batch_run(b, TSW_RUN_BATCHED);
+    if (rc == kOk) rc = tsw_batch_shot_result(b, 0, image_out, illum_out);
+    tsw_batch_destroy(b);
+    return rc;
+  }
   H2D up{st};
   TSW_TRY(upload_model(g, vel_fwd, 0, st));
   TSW_HIP(hipStreamSynchronize(st));  // g->vel is reused for the second model
