@@ -99,8 +99,8 @@ class ShotBatch:
         N.check(N.lib().tsw_batch_run(self._h, m))
 
     def time_kernels(self, lane: str = "acoustic"):
-        """One run of a lane with a HIP event pair around every launch ->
-        (summed kernel milliseconds, launches)."""
+        """One run of a lane with a HIP event pair around each run of 32
+        back-to-back launches -> (summed milliseconds, launches)."""
         ms, n = N.D(), N.I()
         N.check(N.lib().tsw_batch_time_kernels(self._h, 1 if lane == "elastic" else 0,
                                                ctypes.byref(ms), ctypes.byref(n)))

@@ -700,12 +700,14 @@ int tsw_batch_time_kernels(tsw_batch_t h, int lane, double* kernel_ms, int* laun
   }
   TSW_ARG(B->nshot > 0, "batch has no acoustic shots");
   hipStream_t st = B->stream;
-  // event pair around every launch, recorded in chunks to bound the pool
+  // one event pair around each run of kTimedRun back-to-back launches (an
+  // event between every two launches would add its own cost to short ones);
+  // the pairs are drained in chunks to bound the pool
   const int chunk = 512;
   std::vector<hipEvent_t> ev(2 * chunk);
   for (auto& e : ev) TSW_HIP(hipEventCreate(&e));
   double total = 0.0;
-  int count = 0, used = 0;
+  int count = 0, used = 0, inrun = 0;
   auto drain = [&]() -> int {
     TSW_HIP(hipEventSynchronize(ev[2 * used - 1]));
     for (int j = 0; j < used; ++j) {
@@ -723,12 +725,18 @@ int tsw_batch_time_kernels(tsw_batch_t h, int lane, double* kernel_ms, int* laun
       const Epoch* e = epoch_at(B, g);
       if (!e || e->tiles == 0) continue;
     }
-    if (hipEventRecord(ev[2 * used], st) != hipSuccess) rc = kErrHip;
+    if (inrun == 0 && hipEventRecord(ev[2 * used], st) != hipSuccess) rc = kErrHip;
     issue(B, g, g + 1, st);
+    ++count;
+    if (++inrun == kTimedRun) {
+      if (hipEventRecord(ev[2 * used + 1], st) != hipSuccess) rc = kErrHip;
+      inrun = 0;
+      if (++used == chunk && rc == kOk) rc = drain();
+    }
+  }
+  if (rc == kOk && inrun) {
     if (hipEventRecord(ev[2 * used + 1], st) != hipSuccess) rc = kErrHip;
     ++used;
-    ++count;
-    if (used == chunk) rc = drain();
   }
   if (rc == kOk && used) rc = drain();
   if (rc == kOk && B->kernel == 0) rc = finish_shots(B, st);

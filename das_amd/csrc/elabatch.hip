@@ -418,7 +418,7 @@ int ela_lane_time(ElaLane* L, double* ms, int* launches) {
   std::vector<hipEvent_t> ev(2 * chunk);
   for (auto& e : ev) TSW_HIP(hipEventCreate(&e));
   double total = 0.0;
-  int count = 0, used = 0, rc = kOk;
+  int count = 0, used = 0, inrun = 0, rc = kOk;
   auto drain = [&]() -> int {
     TSW_HIP(hipEventSynchronize(ev[2 * used - 1]));
     for (int j = 0; j < used; ++j) {
@@ -433,15 +433,21 @@ int ela_lane_time(ElaLane* L, double* ms, int* launches) {
     const EEpoch* e = e_epoch_at(L, g);
     if (!e || e->tiles == 0) continue;
     for (int pass = 0; pass < 2 && rc == kOk; ++pass) {
-      if (hipEventRecord(ev[2 * used], st) != hipSuccess) rc = kErrHip;
+      if (inrun == 0 && hipEventRecord(ev[2 * used], st) != hipSuccess) rc = kErrHip;
       hipLaunchKernelGGL(k_ela_batch, dim3(e->tiles), dim3(kEBX, kEBZ), 0, st, L->c,
                          (const EShot*)L->d_shots.p, (const int*)e->d_list, e->nshot, e->tiles,
                          g, pass);
-      if (hipEventRecord(ev[2 * used + 1], st) != hipSuccess) rc = kErrHip;
-      ++used;
       ++count;
-      if (used == chunk) rc = drain();
+      if (++inrun == kTimedRun) {
+        if (hipEventRecord(ev[2 * used + 1], st) != hipSuccess) rc = kErrHip;
+        inrun = 0;
+        if (++used == chunk && rc == kOk) rc = drain();
+      }
     }
+  }
+  if (rc == kOk && inrun) {
+    if (hipEventRecord(ev[2 * used + 1], st) != hipSuccess) rc = kErrHip;
+    ++used;
   }
   if (rc == kOk && used) rc = drain();
   if (rc == kOk) rc = e_finish(L, st);

@@ -7,6 +7,9 @@
 
 namespace tsw {
 
+// Launches between the two events of one pair in the kernel-timing runs.
+constexpr int kTimedRun = 32;
+
 // Owning device allocation.
 struct DevBuf {
   void* p = nullptr;

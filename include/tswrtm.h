@@ -199,8 +199,8 @@ int tsw_kernel_config(int id, int* T, int* W, int* H, int* threads);
 /* Migrates every shot (mode TSW_RUN_BATCHED or TSW_RUN_PLAIN); synchronous. */
 int tsw_batch_run(tsw_batch_t batch, int mode);
 /* Runs one lane of the batch (0 acoustic, 1 elastic) once with a HIP event
- * pair around every launch (no graphs); returns the summed kernel time and the
- * number of launches. */
+ * pair around each run of 32 back-to-back launches (no graphs); returns the
+ * summed time of those runs and the number of launches. */
 int tsw_batch_time_kernels(tsw_batch_t batch, int lane, double* kernel_ms, int* launches);
 /* Per-shot image and illumination (nz, nx) of the last run; either may be NULL. */
 int tsw_batch_shot_result(tsw_batch_t batch, int shot, float* image_out, float* illum_out);
