@@ -69,6 +69,15 @@ int tb_config(int id, int* T, int* W, int* H);
 void tb_launch(int id, const BatchConst& bc, const TbShot* shots, const TbEntry* ents,
                const int* pre, int nent, int ntiles, hipStream_t st);
 
+// Global-address-space view of a device pointer.  Loads through a generic
+// pointer become flat loads, which also count against the LDS wait counter, so
+// every LDS wait would wait for them too; global loads do not.
+using gfloat = __attribute__((address_space(1))) float;
+__device__ __forceinline__ const gfloat* as_global(const float* p) { return (const gfloat*)p; }
+__device__ __forceinline__ gfloat* as_global(float* p) { return (gfloat*)p; }
+using gint = __attribute__((address_space(1))) int;
+__device__ __forceinline__ const gint* as_global(const int* p) { return (const gint*)p; }
+
 template <int T, int R, int WX, int WZ>
 struct TbShape {
   static constexpr int CX = 64 * WX;     // extended columns
@@ -82,10 +91,17 @@ struct TbShape {
   static_assert(R >= 3, "rows 0..2 of a free surface must belong to one thread");
 };
 
+// Waves per SIMD the register allocation must leave room for: two 1024-thread
+// workgroups per CU (8 waves per SIMD) for the 4-row shapes, which then fit
+// in 64 VGPRs; no constraint for the others.
+template <int R, int WX, int WZ>
+constexpr int kTbWavesPerEu = R <= 4 && WX * WZ >= 16 ? 8 : 1;
+
 template <int T, int R, int WX, int WZ>
-__global__ __launch_bounds__(64 * WX * WZ) void k_tb(BatchConst bc, const TbShot* shots,
-                                                      const TbEntry* ents, const int* pre,
-                                                      int nent, int ntiles) {
+__global__ __launch_bounds__(64 * WX * WZ)
+__attribute__((amdgpu_waves_per_eu(kTbWavesPerEu<R, WX, WZ>))) void k_tb(
+    BatchConst bc, const TbShot* shots, const TbEntry* ents, const int* pre, int nent,
+    int ntiles) {
   using S = TbShape<T, R, WX, WZ>;
   __shared__ float lds[2][S::LR * S::LS];
 
@@ -130,7 +146,7 @@ __global__ __launch_bounds__(64 * WX * WZ) void k_tb(BatchConst bc, const TbShot
   const unsigned m_fs = (bc.fs && row0 == 0) ? (m_in & 3u) : 0u;
   const unsigned m_keep = m_in & ~m_int & ~m_fs;
 
-  const float* coef = fwd ? P.coef_f : P.coef_b;
+  const gfloat* coef = as_global(fwd ? P.coef_f : P.coef_b);
   float a[R], bl[R], c[R], cf[R], sp[R], acc[R];
 
   // ---- load (or generate) the three input levels
@@ -151,19 +167,19 @@ __global__ __launch_bounds__(64 * WX * WZ) void k_tb(BatchConst bc, const TbShot
       if (d > 0) sp[q] = bc.table[d];
       if (first) {
         if (fwd) {
-          if (idx == P.src_cell) c[q] = 0.0f + P.src[0] * bc.inj;
+          if (idx == P.src_cell) c[q] = 0.0f + as_global(P.src)[0] * bc.inj;
         } else if (row == P.riz) {
-          const int r0 = P.roff[col], r1 = P.roff[col + 1];
-          const float* drow = P.data + (long long)(P.nt - 1) * P.nrec;
+          const int r0 = as_global(P.roff)[col], r1 = as_global(P.roff)[col + 1];
+          const gfloat* drow = as_global(P.data) + (long long)(P.nt - 1) * P.nrec;
           float v = 0.0f;
-          for (int r = r0; r < r1; ++r) v = v + drow[P.ridx[r]] * bc.inj;
+          for (int r = r0; r < r1; ++r) v = v + drow[as_global(P.ridx)[r]] * bc.inj;
           c[q] = v;
         }
       } else {
-        c[q] = P.set[rd][2][idx];
-        bl[q] = P.set[rd][1][idx];
-        if (m_keep >> q & 1u) a[q] = P.set[rd][0][idx];
-        if (m_core >> q & 1u) acc[q] = fwd ? P.illum[idx] : P.image[idx];
+        c[q] = as_global(P.set[rd][2])[idx];
+        bl[q] = as_global(P.set[rd][1])[idx];
+        if (m_keep >> q & 1u) a[q] = as_global(P.set[rd][0])[idx];
+        if (m_core >> q & 1u) acc[q] = as_global(fwd ? P.illum : P.image)[idx];
       }
     }
   }
@@ -172,26 +188,54 @@ __global__ __launch_bounds__(64 * WX * WZ) void k_tb(BatchConst bc, const TbShot
   for (int q = 0; q < R; ++q) X[(ly * R + q + 2) * S::LS + lx + 2] = c[q];
   __syncthreads();
 
+  // Shot fields used every sub-step, held in registers: read through `P`
+  // they would be re-read after each store the compiler cannot disambiguate.
+  const int k = P.k, nt = P.nt, nrec = P.nrec, riz = P.riz;
+  const gfloat* src = as_global(P.src);
+  const gfloat* data = as_global(P.data);
+  gfloat* snaps = as_global(P.snaps);
+  const int src_row = P.src_cell / nx, src_col = P.src_cell - src_row * nx;
+  const bool src_thread = fwd && src_col == col && src_row >= row0 && src_row < row0 + R;
+  // receiver of this column on the back-propagation row: its index when the
+  // column has exactly one, -1 with none, -2 with several
+  int rsel = -1;
+  if (!fwd && riz >= row0 && riz < row0 + R && col_in) {
+    const int r0 = as_global(P.roff)[col], r1 = as_global(P.roff)[col + 1];
+    rsel = r1 - r0 == 1 ? as_global(P.ridx)[r0] : (r1 > r0 ? -2 : -1);
+  }
+  // iteration `it` as phase it % k and snapshot level it / k, stepped by one
+  // per sub-step instead of divided
+  int ph = E.it0 % k, lvl = E.it0 / k;
+
   const float w = bc.w;
   for (int j = 0; j < E.nsub; ++j) {
     const int it = fwd ? E.it0 + j : E.it0 - j;
     float* Y = lds[(j + 1) & 1];
-    const bool imaging = it % P.k == 0;
     // side work on the post-injection level `c`
-    if (imaging && m_core) {
-      float* snap = P.snaps + (long long)(it / P.k) * ncell;
+    if (ph == 0 && m_core) {
+      gfloat* snap = snaps + (long long)lvl * ncell;
+      if (fwd) {
 #pragma unroll
-      for (int q = 0; q < R; ++q) {
-        if (m_core >> q & 1u) {
-          const int idx = (row0 + q) * nx + col;
-          // snapshots stream through HBM once each way: keep them out of the caches
-          if (fwd) {
-            __builtin_nontemporal_store(c[q], snap + idx);
+        for (int q = 0; q < R; ++q) {
+          if (m_core >> q & 1u) {
+            // snapshots stream through HBM once each way: keep them out of the caches
+            const unsigned off = (unsigned)((row0 + q) * nx + col);
+            __builtin_nontemporal_store(c[q], snap + off);
             acc[q] = acc[q] + c[q] * c[q];
-          } else {
-            acc[q] = acc[q] + (__builtin_nontemporal_load(snap + idx) * c[q]) * w;
           }
         }
+      } else {
+        // every row's load is issued before the first use, so the rows share
+        // one memory latency instead of queueing one after another
+        float sv[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const unsigned off = (unsigned)((row0 + q) * nx + col);
+          sv[q] = (m_core >> q & 1u) ? __builtin_nontemporal_load(snap + off) : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          if (m_core >> q & 1u) acc[q] = acc[q] + (sv[q] * c[q]) * w;
       }
     }
     // leapfrog update
@@ -200,26 +244,24 @@ __global__ __launch_bounds__(64 * WX * WZ) void k_tb(BatchConst bc, const TbShot
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int lr = (ly * R + q + 2) * S::LS + lx;
-      float v;
-      if (m_int >> q & 1u) {
-        const float pc = c[q];
-        const float xm2 = xr[lr - 2], xm1 = xr[lr - 1], xp1 = xr[lr + 1], xp2 = xr[lr + 2];
-        const float zm2 = q >= 2 ? c[q - 2] : xr[lr - 2 * S::LS];
-        const float zm1 = q >= 1 ? c[q - 1] : xr[lr - S::LS];
-        const float zp1 = q + 1 < R ? c[q + 1] : xr[lr + S::LS];
-        const float zp2 = q + 2 < R ? c[q + 2] : xr[lr + 2 * S::LS];
-        float lx2 = TSW_C2 * (xm2 + xp2);
-        lx2 = lx2 + TSW_C1 * (xm1 + xp1);
-        lx2 = (lx2 + TSW_C0 * pc) * bc.rdx2;
-        float lz2 = TSW_C2 * (zm2 + zp2);
-        lz2 = lz2 + TSW_C1 * (zm1 + zp1);
-        lz2 = (lz2 + TSW_C0 * pc) * bc.rdz2;
-        v = (2.0f * pc - bl[q]) + cf[q] * (lx2 + lz2);
-        v = v * sp[q];
-      } else {
-        v = a[q];  // kept cells (and cells outside the grid, which stay 0)
-      }
-      n[q] = v;
+      // evaluated on every lane (all reads stay inside this LDS buffer), so the
+      // reads of all rows issue together; non-interior lanes then take `a`
+      const float pc = c[q];
+      const float xm2 = xr[lr - 2], xm1 = xr[lr - 1], xp1 = xr[lr + 1], xp2 = xr[lr + 2];
+      const float zm2 = q >= 2 ? c[q - 2] : xr[lr - 2 * S::LS];
+      const float zm1 = q >= 1 ? c[q - 1] : xr[lr - S::LS];
+      const float zp1 = q + 1 < R ? c[q + 1] : xr[lr + S::LS];
+      const float zp2 = q + 2 < R ? c[q + 2] : xr[lr + 2 * S::LS];
+      float lx2 = TSW_C2 * (xm2 + xp2);
+      lx2 = lx2 + TSW_C1 * (xm1 + xp1);
+      lx2 = (lx2 + TSW_C0 * pc) * bc.rdx2;
+      float lz2 = TSW_C2 * (zm2 + zp2);
+      lz2 = lz2 + TSW_C1 * (zm1 + zp1);
+      lz2 = (lz2 + TSW_C0 * pc) * bc.rdz2;
+      float v = (2.0f * pc - bl[q]) + cf[q] * (lx2 + lz2);
+      v = v * sp[q];
+      // kept cells (and cells outside the grid, which stay 0) keep `a`
+      n[q] = (m_int >> q & 1u) ? v : a[q];
     }
     if (m_fs) {  // this thread holds rows 0, 1 and 2 of its column
       n[0] = 0.0f;
@@ -227,23 +269,36 @@ __global__ __launch_bounds__(64 * WX * WZ) void k_tb(BatchConst bc, const TbShot
     }
     // injection of the next iteration's samples
     if (fwd) {
-      if (it + 1 < P.nt) {
-        const int sr = P.src_cell / nx, sc = P.src_cell - sr * nx;
-        if (sc == col && sr >= row0 && sr < row0 + R) {
-#pragma unroll
-          for (int q = 0; q < R; ++q)
-            if (row0 + q == sr) n[q] = n[q] + P.src[it + 1] * bc.inj;
-        }
-      }
-    } else if (it >= 1 && P.riz >= row0 && P.riz < row0 + R && col_in) {
-      const int r0 = P.roff[col], r1 = P.roff[col + 1];
-      if (r0 < r1) {
-        const float* drow = P.data + (long long)(it - 1) * P.nrec;
+      if (src_thread && it + 1 < nt) {
+        const float s = src[it + 1];
 #pragma unroll
         for (int q = 0; q < R; ++q)
-          if (row0 + q == P.riz)
-            for (int r = r0; r < r1; ++r) n[q] = n[q] + drow[P.ridx[r]] * bc.inj;
+          if (row0 + q == src_row) n[q] = n[q] + s * bc.inj;
       }
+    } else if (rsel != -1 && it >= 1) {
+      const gfloat* drow = data + (long long)(it - 1) * nrec;
+      if (rsel >= 0) {
+        const float d = drow[rsel];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          if (row0 + q == riz) n[q] = n[q] + d * bc.inj;
+      } else {
+        const int r0 = as_global(P.roff)[col], r1 = as_global(P.roff)[col + 1];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          if (row0 + q == riz)
+            for (int r = r0; r < r1; ++r) n[q] = n[q] + drow[as_global(P.ridx)[r]] * bc.inj;
+      }
+    }
+    // next iteration's phase and snapshot level
+    if (fwd) {
+      if (++ph == k) {
+        ph = 0;
+        ++lvl;
+      }
+    } else if (ph-- == 0) {
+      ph = k - 1;
+      --lvl;
     }
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -263,12 +318,12 @@ __global__ __launch_bounds__(64 * WX * WZ) void k_tb(BatchConst bc, const TbShot
     if (m_core >> q & 1u) {
       const int idx = (row0 + q) * nx + col;
       if (!last) {
-        P.set[wr][2][idx] = c[q];
-        P.set[wr][1][idx] = bl[q];
-        if (m_keep >> q & 1u) P.set[wr][0][idx] = a[q];
+        as_global(P.set[wr][2])[idx] = c[q];
+        as_global(P.set[wr][1])[idx] = bl[q];
+        if (m_keep >> q & 1u) as_global(P.set[wr][0])[idx] = a[q];
       }
-      if (fwd) P.illum[idx] = last ? acc[q] * w : acc[q];
-      else P.image[idx] = acc[q];
+      if (fwd) as_global(P.illum)[idx] = last ? acc[q] * w : acc[q];
+      else as_global(P.image)[idx] = acc[q];
     }
   }
 }
