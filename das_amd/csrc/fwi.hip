@@ -21,12 +21,16 @@
 // so each step reads Lam^{n+1} (stencil), Lam^{n+2} and, on the border cells
 // only, Lam^{n+3}, and writes Lam^n: four buffers rotate with period 4.  L is
 // symmetric (central stencil), so L^T is the same stencil applied to the
-// zero-extended field C.w, which each block stages in LDS with its halo.
+// zero-extended field C.w, which each block stages in LDS with its halo
+// (with w and S, which the block's own cells then read from LDS).
 //
 // The forward levels u^n are not stored: every ceil(sqrt(nt)) steps the three
 // levels (u^{n-2}, u^{n-1}, u^n) are checkpointed, and the backward sweep
 // recomputes each segment from its checkpoint (bit-identical to the first
-// pass) before running that segment's adjoint steps.
+// pass) before running that segment's adjoint steps.  The recompute keeps,
+// for every step of the segment, L u^n on the interior (the Laplacian the
+// step computes anyway) rather than u^n: the gradient term then reads one
+// value per cell instead of a 9-point stencil.
 #include <cmath>
 #include <type_traits>
 #include <cstring>
@@ -70,7 +74,7 @@ struct FwdArgs {
   const T* cur;
   const T* prv;
   T* nxt;
-  T* save;           // non-null: save + j * n = cur (u^n)
+  T* save;           // non-null: save + j * n = L cur (L u^n) on the interior, 0 elsewhere
   const FwiMeta* meta;
   int src_it, rec_it;  // -1: none
 };
@@ -145,12 +149,14 @@ __global__ __launch_bounds__(256) void k_fwi_fwd(FwdArgs<T> s) {
     const int c = iz * nx + ix;
     const int l = (lz + 2) * kLW + threadIdx.x + 2;
     const T pc = tile[l];
-    if (s.save) s.save[o + c] = pc;
-    if (s.fs && iz < 2) continue;
     const bool interior = iz >= 2 && iz < nz - 2 && ix >= 2 && ix < nx - 2;
+    // the gradient needs L u^n on the interior: saved instead of u^n (0 elsewhere)
+    const T lap = interior ? lap4_lds(tile, l, s.rdx2, s.rdz2) : T(0);
+    if (s.save) s.save[o + c] = lap;
+    if (s.fs && iz < 2) continue;
     T v;
     if (interior) {
-      v = (T(2) * pc - s.prv[o + c]) + s.C[c] * lap4_lds(tile, l, s.rdx2, s.rdz2);
+      v = (T(2) * pc - s.prv[o + c]) + s.C[c] * lap;
       v = v * s.S[c];
     } else {
       v = nxt[c];
@@ -191,14 +197,14 @@ struct AdjArgs {
   const T* lam2;  // Lam^{n+2}
   const T* lam3;  // Lam^{n+3}
   T* lam0;        // Lam^n (written)
-  const T* un;    // forward u^n
+  const T* lun;   // forward L u^n (the Laplacian the forward step used; saved by the recompute)
   T* grad;        // g_C, one slice per shot of the group
   const FwiMeta* meta;
   int res_it;     // residual row added to Lam^n (n-1), -1: none
   int probe_it;
   // two-step kernel: Lam^{n-1} too
   T* lamm1;       // Lam^{n-1} (written)
-  const T* um1;   // forward u^{n-1}
+  const T* lum1;  // forward L u^{n-1}
   int res_it2;    // residual row added to Lam^{n-1} (n-2), -1: none
 };
 
@@ -224,29 +230,91 @@ __device__ __forceinline__ T fwi_sponge(const AdjArgs<T>& a, int iz, int ix, int
   return d > 0 ? (T)a.table[d] : T(1);
 }
 
+// An empty asm that takes and gives back `v` in a register: arithmetic on `v`
+// after it cannot be moved up next to the read that produced it.
+template <typename T>
+__device__ __forceinline__ void opaque(T& v) {
+  asm volatile("" : "+v"(v));
+}
+
 template <typename T, int TZ>
 __global__ __launch_bounds__(256) void k_fwi_adj(AdjArgs<T> a) {
   constexpr int LH = TZ + 4;
-  __shared__ T cw[LH * kLW];  // C.w^{n+1}, zero outside I
+  constexpr int NL = LH * kLW;
+  constexpr int NST = (NL + 255) / 256;  // staging passes of the 256 threads
+  __shared__ T cw[NL];  // C.w^{n+1}, zero outside I
+  __shared__ T wl[NL];  // w^{n+1} = S.zeta(Lam^{n+1})
+  __shared__ T sl[NL];  // S
   const int nx = a.nx, nz = a.nz;
   const int x0 = blockIdx.x * 64, z0 = blockIdx.y * TZ;
   const long long o = (long long)blockIdx.z * a.n;
   const T* lam1 = a.lam1 + o;
-  for (int i = threadIdx.y * 64 + threadIdx.x; i < LH * kLW; i += 256) {
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  // Staging: every pass's reads are issued before any is used, so the passes
+  // share one memory latency instead of one each.
+  T cv[NST], lv[NST], sv[NST];
+#pragma unroll
+  for (int p = 0; p < NST; ++p) {
+    const int i = tid + 256 * p;
     const int gz = z0 - 2 + i / kLW, gx = x0 - 2 + i % kLW;
-    T v = T(0);
-    if (fwi_interior<T>(gz, gx, nz, nx)) {
+    cv[p] = lv[p] = sv[p] = T(0);
+    if (i < NL && fwi_interior<T>(gz, gx, nz, nx)) {
       const int c = gz * nx + gx;
-      v = a.C[c] * (fwi_sponge(a, gz, gx, c) * fwi_zeta(a.fs, lam1, c, gz, nx));
+      cv[p] = a.C[c];
+      lv[p] = lam1[c];
+      // table[0] = 1, so the table is read for every depth
+      sv[p] = a.table ? (T)a.table[sponge_depth(gx, gz, nx, nz, a.n_sponge, a.fs)] : a.S[c];
     }
-    cw[i] = v;
+  }
+#pragma unroll
+  for (int p = 0; p < NST; ++p) {
+    opaque(cv[p]);
+    opaque(lv[p]);
+    opaque(sv[p]);
+  }
+#pragma unroll
+  for (int p = 0; p < NST; ++p) {
+    const int i = tid + 256 * p;
+    if (i < NL) {
+      const int gz = z0 - 2 + i / kLW, gx = x0 - 2 + i % kLW;
+      // zeta (Q^T) on the interior: the free surface changes row 2 only
+      T z = lv[p];
+      if (a.fs && gz == 2 && fwi_interior<T>(gz, gx, nz, nx)) z = z - lam1[gz * nx + gx - nx];
+      const T w = sv[p] * z;  // 0 outside I (all factors read as 0 there)
+      cw[i] = cv[p] * w;
+      wl[i] = w;
+      sl[i] = sv[p];
+    }
   }
   __syncthreads();
   const int ix = x0 + threadIdx.x;
   if (ix >= nx) return;
   const FwiMeta& m = a.meta[blockIdx.z];
   const T c0 = FwiK<T>::c0, c1 = fwi_c1<T>(), c2 = fwi_c2<T>();
-  for (int k = 0; k < TZ / 4; ++k) {
+  constexpr int KR = TZ / 4;  // rows per thread, 4 apart
+  // the interior cells' remaining reads of all rows, issued together
+  const T* lam2 = a.lam2 + o;
+  const T* lun = a.lun + o;
+  T* grad = a.grad + o;
+  // (read for every row, the row clamped into the grid, so no row branches;
+  // only interior cells use them)
+  T l2[KR], g[KR], lu[KR];
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int iz = min(z0 + (int)threadIdx.y + 4 * k, nz - 1);
+    const unsigned c = (unsigned)(iz * nx + ix);
+    l2[k] = lam2[c];
+    g[k] = grad[c];
+    lu[k] = lun[c];
+  }
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    opaque(l2[k]);
+    opaque(g[k]);
+    opaque(lu[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
     const int lz = threadIdx.y + 4 * k, iz = z0 + lz;
     if (iz >= nz) break;
     const int c = iz * nx + ix;
@@ -260,11 +328,12 @@ __global__ __launch_bounds__(256) void k_fwi_adj(AdjArgs<T> a) {
     lz2 = (lz2 + c0 * cw[l]) * a.rdz2;
     T acc;
     if (fwi_interior<T>(iz, ix, nz, nx)) {
-      const T sc = fwi_sponge(a, iz, ix, c);
-      const T w1 = sc * fwi_zeta(a.fs, lam1, c, iz, nx);
-      const T w2 = sc * fwi_zeta(a.fs, a.lam2 + o, c, iz, nx);
+      const T w1 = wl[l];
+      T z2 = l2[k];
+      if (a.fs && iz == 2) z2 = z2 - lam2[c - nx];
+      const T w2 = sl[l] * z2;
       acc = (T(2) * w1 + (lx + lz2)) - w2;
-      a.grad[o + c] = a.grad[o + c] + w1 * lap4(a.un + o, c, nx, a.rdx2, a.rdz2);
+      grad[c] = g[k] + w1 * lu[k];
     } else {
       acc = lx + lz2;
       if (!(a.fs && iz < 2)) acc = acc + fwi_zeta(a.fs, a.lam3 + o, c, iz, nx);
@@ -285,7 +354,7 @@ __global__ __launch_bounds__(256) void k_fwi_adj(AdjArgs<T> a) {
 // Lam^{n-1} on the tile.  The step formulas, their FP order and the order of
 // the two gradient additions are those of k_fwi_adj applied at n then n-1, so
 // the result is bit-identical to two single steps.  Reads Lam^{n+1} (tile +
-// 4), Lam^{n+2}, Lam^{n+3} (border cells), u^n, u^{n-1}; writes Lam^n,
+// 4), Lam^{n+2}, Lam^{n+3} (border cells), L u^n, L u^{n-1}; writes Lam^n,
 // Lam^{n-1} into buffers no workgroup of the launch reads (five rotate).
 template <typename T, int TZ>
 __global__ __launch_bounds__(256) void k_fwi_adj2(AdjArgs<T> a) {
@@ -384,8 +453,8 @@ __global__ __launch_bounds__(256) void k_fwi_adj2(AdjArgs<T> a) {
       const T wp = sc * fwi_zeta(a.fs, lam1, c, iz, nx);
       acc = (T(2) * w0 + (lx + lz2)) - wp;
       T g = a.grad[o + c];
-      g = g + wp * lap4(a.un + o, c, nx, a.rdx2, a.rdz2);
-      g = g + w0 * lap4(a.um1 + o, c, nx, a.rdx2, a.rdz2);
+      g = g + wp * a.lun[o + c];
+      g = g + w0 * a.lum1[o + c];
       a.grad[o + c] = g;
     } else {
       acc = lx + lz2;
@@ -495,7 +564,7 @@ struct FwiTbArgs {
   long long n;               // cells per shot
   const float* rd[3];        // set read: u^{n-2}, u^{n-1}, u^n (shot j at + j * n)
   float* wr[3];              // set written
-  float* save;               // non-null: save + (it - save_it0) * sstride + j * n = u^it
+  float* save;               // non-null: save + (it - save_it0) * sstride + j * n = L u^it
   long long sstride;
   int save_it0;
   const FwiMeta* meta;
@@ -555,59 +624,60 @@ __global__ __launch_bounds__(64 * WX * WZ) void k_fwi_tb(FwiTbArgs a) {
   for (int q = 0; q < R; ++q) X[(ly * R + q + 2) * S::LS + lx + 2] = c[q];
   __syncthreads();
   const int src_row = m.src_cell / nx, src_col = m.src_cell - src_row * nx;
+  const bool src_thread = src_col == col && src_row >= row0 && src_row < row0 + R;
+  // receivers of this column recorded from this thread's rows (read once)
+  int rec_r0 = 0, rec_r1 = 0;
+  if (a.rec && m.riz >= row0 && m.riz < row0 + R && core_col) {
+    rec_r0 = as_global(m.roff)[col];
+    rec_r1 = as_global(m.roff)[col + 1];
+  }
   for (int j = 0; j < a.nsub; ++j) {
     const int it = a.it0 + j;
     float* Y = lds[(j + 1) & 1];
-    if (a.save && m_core) {
-      float* sv = a.save + (long long)(it - a.save_it0) * a.sstride + o;
-#pragma unroll
-      for (int q = 0; q < R; ++q)
-        if (m_core >> q & 1u) sv[(row0 + q) * nx + col] = c[q];
-    }
+    // the gradient needs L u^it on the interior: saved instead of u^it (0 elsewhere)
+    float* sv = a.save ? a.save + (long long)(it - a.save_it0) * a.sstride + o : nullptr;
     float nv[R];
     const float* xr = X + 2;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int lr = (ly * R + q + 2) * S::LS + lx;
-      float v;
-      if (m_int >> q & 1u) {
-        const float pc = c[q];
-        const float xm2 = xr[lr - 2], xm1 = xr[lr - 1], xp1 = xr[lr + 1], xp2 = xr[lr + 2];
-        const float zm2 = q >= 2 ? c[q - 2] : xr[lr - 2 * S::LS];
-        const float zm1 = q >= 1 ? c[q - 1] : xr[lr - S::LS];
-        const float zp1 = q + 1 < R ? c[q + 1] : xr[lr + S::LS];
-        const float zp2 = q + 2 < R ? c[q + 2] : xr[lr + 2 * S::LS];
-        float lxx = TSW_C2 * (xm2 + xp2);
-        lxx = lxx + TSW_C1 * (xm1 + xp1);
-        lxx = (lxx + TSW_C0 * pc) * a.rdx2;
-        float lzz = TSW_C2 * (zm2 + zp2);
-        lzz = lzz + TSW_C1 * (zm1 + zp1);
-        lzz = (lzz + TSW_C0 * pc) * a.rdz2;
-        v = (2.0f * pc - u1[q]) + cf[q] * (lxx + lzz);
-        v = v * sp[q];
-      } else {
-        v = u2[q];  // cells the step never writes keep the level three buffers back
-      }
-      nv[q] = v;
+      // evaluated on every lane (all reads stay inside this LDS buffer), so the
+      // reads of all rows can issue together
+      const float pc = c[q];
+      const float xm2 = xr[lr - 2], xm1 = xr[lr - 1], xp1 = xr[lr + 1], xp2 = xr[lr + 2];
+      const float zm2 = q >= 2 ? c[q - 2] : xr[lr - 2 * S::LS];
+      const float zm1 = q >= 1 ? c[q - 1] : xr[lr - S::LS];
+      const float zp1 = q + 1 < R ? c[q + 1] : xr[lr + S::LS];
+      const float zp2 = q + 2 < R ? c[q + 2] : xr[lr + 2 * S::LS];
+      float lxx = TSW_C2 * (xm2 + xp2);
+      lxx = lxx + TSW_C1 * (xm1 + xp1);
+      lxx = (lxx + TSW_C0 * pc) * a.rdx2;
+      float lzz = TSW_C2 * (zm2 + zp2);
+      lzz = lzz + TSW_C1 * (zm1 + zp1);
+      lzz = (lzz + TSW_C0 * pc) * a.rdz2;
+      const float lap = lxx + lzz;
+      if (sv && (m_core >> q & 1u)) sv[(row0 + q) * nx + col] = (m_int >> q & 1u) ? lap : 0.0f;
+      float v = (2.0f * pc - u1[q]) + cf[q] * lap;
+      v = v * sp[q];
+      // cells the step never writes keep the level three buffers back
+      nv[q] = (m_int >> q & 1u) ? v : u2[q];
     }
     if (m_fs) {  // rows 0, 1, 2 of this column belong to this thread
       nv[0] = 0.0f;
       nv[1] = -nv[2];
     }
-    if (a.rec && m.riz >= row0 && m.riz < row0 + R && core_col) {
-      const int r0 = m.roff[col], r1 = m.roff[col + 1];
-      if (r0 < r1) {
-        float* rec = (float*)m.pred + (long long)it * m.nrec;
-#pragma unroll
-        for (int q = 0; q < R; ++q)
-          if (row0 + q == m.riz && (m_core >> q & 1u))
-            for (int r = r0; r < r1; ++r) rec[m.ridx[r]] = nv[q];
-      }
-    }
-    if (it + 1 < a.nt && src_col == col && src_row >= row0 && src_row < row0 + R) {
+    if (rec_r0 < rec_r1) {
+      gfloat* rec = as_global((float*)m.pred) + (long long)it * m.nrec;
 #pragma unroll
       for (int q = 0; q < R; ++q)
-        if (row0 + q == src_row) nv[q] = nv[q] + ((const float*)m.src)[it + 1] * a.inj;
+        if (row0 + q == m.riz && (m_core >> q & 1u))
+          for (int r = rec_r0; r < rec_r1; ++r) rec[as_global(m.ridx)[r]] = nv[q];
+    }
+    if (src_thread && it + 1 < a.nt) {
+      const float s = as_global((const float*)m.src)[it + 1];
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+        if (row0 + q == src_row) nv[q] = nv[q] + s * a.inj;
     }
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -768,7 +838,7 @@ static FwiTbArgs tb_base(const Fwi* F, const FwiGroup& G) {
 }
 
 // Blocked steps [i0, i1) of a group from state set *k (updated), recording
-// the receivers or saving every level u^it into `save` (segment starting at
+// the receivers or saving L u^it of every step into `save` (segment starting at
 // save_it0); blocks never cross a multiple of `cut` (checkpoint boundaries,
 // 0: none) so that the state is in memory there.
 static void tb_run(const Fwi* F, const FwiGroup& G, int i0, int i1, int* k, bool rec, float* save,
@@ -935,12 +1005,12 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
         a.lam1 = adj<T>(F, it + 1);
         a.lam2 = adj<T>(F, it + 2);
         a.lam3 = adj<T>(F, it + 3);
-        a.un = (const T*)F->seg.p + (long long)(it - n0) * cn;
+        a.lun = (const T*)F->seg.p + (long long)(it - n0) * cn;
         a.res_it = it >= 1 ? it - 1 : -1;
         a.probe_it = it;
         if (F->adj_steps == 2 && it - 1 >= n0) {
           a.lamm1 = adj<T>(F, it + 4);  // Lam^{it-1}: the ring slot of Lam^{it+4}
-          a.um1 = (const T*)F->seg.p + (long long)(it - 1 - n0) * cn;
+          a.lum1 = (const T*)F->seg.p + (long long)(it - 1 - n0) * cn;
           a.res_it2 = it >= 2 ? it - 2 : -1;
           hipLaunchKernelGGL((k_fwi_adj2<T, 16>), dim3((F->nx + 63) / 64, (F->nz + 15) / 16, cnt),
                              dim3(64, 4), 0, st, a);
