@@ -53,6 +53,16 @@ int fail_injected();  // consumes a pending injected failure (0 if none)
 #define TSW_S1 (9.0f / 8.0f)
 #define TSW_S2 (-1.0f / 24.0f)
 
+// Global-address-space view of a device pointer.  Loads through a generic
+// pointer (one read from a descriptor in memory) become flat loads, which
+// also count against the LDS wait counter, so every LDS wait would wait for
+// them too; global loads do not.
+using gfloat = __attribute__((address_space(1))) float;
+__device__ __forceinline__ const gfloat* as_global(const float* p) { return (const gfloat*)p; }
+__device__ __forceinline__ gfloat* as_global(float* p) { return (gfloat*)p; }
+using gint = __attribute__((address_space(1))) int;
+__device__ __forceinline__ const gint* as_global(const int* p) { return (const gint*)p; }
+
 // Integer depth of cell (ix, iz) into the Cerjan strip (wave2d.c:247-256).
 __host__ __device__ inline int sponge_depth(int ix, int iz, int nx, int nz, int n, int fs) {
   int d = 0;

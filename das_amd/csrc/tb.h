@@ -69,15 +69,6 @@ int tb_config(int id, int* T, int* W, int* H);
 void tb_launch(int id, const BatchConst& bc, const TbShot* shots, const TbEntry* ents,
                const int* pre, int nent, int ntiles, hipStream_t st);
 
-// Global-address-space view of a device pointer.  Loads through a generic
-// pointer become flat loads, which also count against the LDS wait counter, so
-// every LDS wait would wait for them too; global loads do not.
-using gfloat = __attribute__((address_space(1))) float;
-__device__ __forceinline__ const gfloat* as_global(const float* p) { return (const gfloat*)p; }
-__device__ __forceinline__ gfloat* as_global(float* p) { return (gfloat*)p; }
-using gint = __attribute__((address_space(1))) int;
-__device__ __forceinline__ const gint* as_global(const int* p) { return (const gint*)p; }
-
 template <int T, int R, int WX, int WZ>
 struct TbShape {
   static constexpr int CX = 64 * WX;     // extended columns
