@@ -17,6 +17,9 @@ namespace tsw {
 
 constexpr int kEBX = 64;
 constexpr int kEBZ = 4;
+// rows per thread, kEBZ apart: a tile is kEBX x (kEBZ * kERows) cells
+constexpr int kERows = 2;
+constexpr int kETileZ = kEBZ * kERows;
 
 __device__ __forceinline__ int xcd_tile_e(int b, int n) {
   const int per = n >> 3, rem = n & 7;
@@ -39,18 +42,22 @@ __global__ __launch_bounds__(kEBX* kEBZ) void k_ela_batch(EConst c, const EShot*
   const EShot& S = shots[list[lo]];
   const int t = b - pre[lo];
   const int ix = (t % S.tiles_x) * kEBX + threadIdx.x;
-  const int iz = (t / S.tiles_x) * kEBZ + threadIdx.y;
-  if (ix >= S.nx || iz >= S.nz) return;
-  const int i = iz * S.nx + ix;
+  const int iz0 = (t / S.tiles_x) * kETileZ + threadIdx.y;
+  if (ix >= S.nx || iz0 >= S.nz) return;
   const int m = g - S.start;
   if (m < 0) {  // initialisation step of the shot
     if (pass == 0) {
-      for (int j = 0; j < 5; ++j) {
-        S.f[j][i] = 0.0f;
-        S.fb[j][i] = 0.0f;
+      for (int r = 0; r < kERows; ++r) {
+        const int iz = iz0 + r * kEBZ;
+        if (iz >= S.nz) break;
+        const int i = iz * S.nx + ix;
+        for (int j = 0; j < 5; ++j) {
+          S.f[j][i] = 0.0f;
+          S.fb[j][i] = 0.0f;
+        }
+        S.illum[i] = 0.0f;
+        S.image[i] = 0.0f;
       }
-      S.illum[i] = 0.0f;
-      S.image[i] = 0.0f;
     }
     return;
   }
@@ -111,8 +118,24 @@ __global__ __launch_bounds__(kEBX* kEBZ) void k_ela_batch(EConst c, const EShot*
   s.mu = model[1];
   s.bx = model[2];
   s.bz = model[3];
-  if (pass == 0) ela_vel_cell(s, ix, iz);
-  else ela_str_cell(s, ix, iz);
+  // every row's reads first, then the rows' updates (see ela_vel_load)
+  if (pass == 0) {
+    ElaVelIn in[kERows];
+#pragma unroll
+    for (int r = 0; r < kERows; ++r)
+      if (iz0 + r * kEBZ < S.nz) in[r] = ela_vel_load(s, ix, iz0 + r * kEBZ);
+#pragma unroll
+    for (int r = 0; r < kERows; ++r)
+      if (iz0 + r * kEBZ < S.nz) ela_vel_finish(s, ix, iz0 + r * kEBZ, in[r]);
+  } else {
+    ElaStrIn in[kERows];
+#pragma unroll
+    for (int r = 0; r < kERows; ++r)
+      if (iz0 + r * kEBZ < S.nz) in[r] = ela_str_load(s, ix, iz0 + r * kEBZ);
+#pragma unroll
+    for (int r = 0; r < kERows; ++r)
+      if (iz0 + r * kEBZ < S.nz) ela_str_finish(s, ix, iz0 + r * kEBZ, in[r]);
+  }
 }
 
 void ela_batch_launch(const EConst& c, const EShot* shots, const int* list, int nact,
@@ -123,7 +146,7 @@ void ela_batch_launch(const EConst& c, const EShot* shots, const int* list, int 
 
 int ela_tiles(int nx, int nz, int* tiles_x) {
   *tiles_x = (nx + kEBX - 1) / kEBX;
-  return *tiles_x * ((nz + kEBZ - 1) / kEBZ);
+  return *tiles_x * ((nz + kETileZ - 1) / kETileZ);
 }
 
 __global__ void k_ela_scale(float* p, long long n, float w) {
@@ -291,7 +314,7 @@ int ela_lane_create(const std::vector<const tsw_shot_desc*>& shots, double dx, d
     S.riz = d.riz;
     S.nrec = d.nrec;
     S.tiles_x = (d.nx + kEBX - 1) / kEBX;
-    S.tiles = S.tiles_x * ((d.nz + kEBZ - 1) / kEBZ);
+    S.tiles = S.tiles_x * ((d.nz + kETileZ - 1) / kETileZ);
     L->cell_updates += 2LL * d.nt * d.nx * d.nz;
   }
   if ((rc = e_build(L))) return fail(rc);

@@ -49,55 +49,73 @@ __device__ __forceinline__ float ela_sponge(const ElaStep& s, int ix, int iz, in
 }
 
 // Velocity pass at one cell, with the driver's injection and side work on the
-// cell's vz before its update (wave2d.c:123-155).  Every read of the update is
-// issued first (the side work writes only the snapshot, illumination and
-// image arrays), through global-address-space pointers.
-__device__ __forceinline__ void ela_vel_cell(const ElaStep& s, int ix, int iz) {
+// cell's vz before its update (wave2d.c:123-155), in two halves: every read of
+// the update (ela_vel_load), then the side work, the update and the writes
+// (ela_vel_finish).  The side work writes only the snapshot, illumination and
+// image arrays, so a thread can issue the reads of several cells before
+// finishing any.  Reads go through global-address-space pointers.
+struct ElaVelIn {
+  bool upd;  // interior: the cell is updated
+  float vz0, bzi, vxi, bxi, g;
+  float sxx_p1, sxx_0, sxx_p2, sxx_m1;
+  float sxz_0, sxz_mz, sxz_pz, sxz_m2z, sxz_m1, sxz_p1, sxz_m2;
+  float szz_pz, szz_0, szz_p2z, szz_mz;
+};
+
+__device__ __forceinline__ ElaVelIn ela_vel_load(const ElaStep& s, int ix, int iz) {
   const int nx = s.nx;
   const int i = iz * nx + ix;
   const int z0 = s.fs ? 1 : 2;
-  const bool upd = !(ix < 2 || ix >= nx - 2 || iz < z0 || iz >= s.nz - 2);
+  ElaVelIn v;
+  v.upd = !(ix < 2 || ix >= nx - 2 || iz < z0 || iz >= s.nz - 2);
   const gfloat* sxx = as_global((const float*)s.sxx);
   const gfloat* szz = as_global((const float*)s.szz);
   const gfloat* sxz = as_global((const float*)s.sxz);
-  const float vz0 = as_global((const float*)s.vz)[i];
-  const float bzi = as_global(s.bz)[i];
-  float vxi = 0.0f, bxi = 0.0f, g = 1.0f;
-  float sxx_p1 = 0.0f, sxx_0 = 0.0f, sxx_p2 = 0.0f, sxx_m1 = 0.0f;
-  float sxz_0 = 0.0f, sxz_mz = 0.0f, sxz_pz = 0.0f, sxz_m2z = 0.0f;
-  float sxz_m1 = 0.0f, sxz_p1 = 0.0f, sxz_m2 = 0.0f;
-  float szz_pz = 0.0f, szz_0 = 0.0f, szz_p2z = 0.0f, szz_mz = 0.0f;
-  if (upd) {
-    vxi = as_global((const float*)s.vx)[i];
-    bxi = as_global(s.bx)[i];
-    sxx_p1 = sxx[i + 1];
-    sxx_0 = sxx[i];
-    sxx_p2 = sxx[i + 2];
-    sxx_m1 = sxx[i - 1];
-    sxz_0 = sxz[i];
-    sxz_mz = sxz[i - nx];
-    sxz_pz = sxz[i + nx];
-    sxz_m2z = i - 2 * nx < 0 ? 0.0f : sxz[i - 2 * nx];  // rows above the grid read 0
-    sxz_m1 = sxz[i - 1];
-    sxz_p1 = sxz[i + 1];
-    sxz_m2 = sxz[i - 2];
-    szz_pz = szz[i + nx];
-    szz_0 = szz[i];
-    szz_p2z = szz[i + 2 * nx];
-    szz_mz = szz[i - nx];
-    g = ela_sponge(s, ix, iz, i);
+  v.vz0 = as_global((const float*)s.vz)[i];
+  v.bzi = as_global(s.bz)[i];
+  v.vxi = v.bxi = 0.0f;
+  v.g = 1.0f;
+  v.sxx_p1 = v.sxx_0 = v.sxx_p2 = v.sxx_m1 = 0.0f;
+  v.sxz_0 = v.sxz_mz = v.sxz_pz = v.sxz_m2z = v.sxz_m1 = v.sxz_p1 = v.sxz_m2 = 0.0f;
+  v.szz_pz = v.szz_0 = v.szz_p2z = v.szz_mz = 0.0f;
+  if (v.upd) {
+    v.vxi = as_global((const float*)s.vx)[i];
+    v.bxi = as_global(s.bx)[i];
+    v.sxx_p1 = sxx[i + 1];
+    v.sxx_0 = sxx[i];
+    v.sxx_p2 = sxx[i + 2];
+    v.sxx_m1 = sxx[i - 1];
+    v.sxz_0 = sxz[i];
+    v.sxz_mz = sxz[i - nx];
+    v.sxz_pz = sxz[i + nx];
+    v.sxz_m2z = i - 2 * nx < 0 ? 0.0f : sxz[i - 2 * nx];  // rows above the grid read 0
+    v.sxz_m1 = sxz[i - 1];
+    v.sxz_p1 = sxz[i + 1];
+    v.sxz_m2 = sxz[i - 2];
+    v.szz_pz = szz[i + nx];
+    v.szz_0 = szz[i];
+    v.szz_p2z = szz[i + 2 * nx];
+    v.szz_mz = szz[i - nx];
+    v.g = ela_sponge(s, ix, iz, i);
   }
-  float vzc = vz0;
+  return v;
+}
+
+__device__ __forceinline__ void ela_vel_finish(const ElaStep& s, int ix, int iz,
+                                               const ElaVelIn& v) {
+  const int nx = s.nx;
+  const int i = iz * nx + ix;
+  float vzc = v.vz0;
   bool injected = false;
   if (s.src_it >= 0 && i == s.src_cell) {
-    vzc = vzc + (s.dt * bzi) * as_global(s.src)[s.src_it];
+    vzc = vzc + (s.dt * v.bzi) * as_global(s.src)[s.src_it];
     injected = true;
   }
   if (s.rdata_it >= 0 && iz == s.riz) {
     const int r0 = as_global(s.roff)[ix], r1 = as_global(s.roff)[ix + 1];
     const gfloat* row = as_global(s.rdata) + (long long)s.rdata_it * s.nrec;
     for (int r = r0; r < r1; ++r) {
-      vzc = vzc + (s.dt * bzi) * row[as_global(s.ridx)[r]];
+      vzc = vzc + (s.dt * v.bzi) * row[as_global(s.ridx)[r]];
       injected = true;
     }
   }
@@ -110,21 +128,25 @@ __device__ __forceinline__ void ela_vel_cell(const ElaStep& s, int ix, int iz) {
     gfloat* im = as_global(s.image);
     im[i] = im[i] + (as_global(s.isnap)[i] * vzc) * s.w;
   }
-  if (!upd) {
+  if (!v.upd) {
     if (injected) as_global(s.vz)[i] = vzc;
     return;
   }
   const float a = TSW_S1, b = TSW_S2;
-  const float dsxx = (a * (sxx_p1 - sxx_0) + b * (sxx_p2 - sxx_m1)) * s.rdx;
-  const float dsxz_z = (a * (sxz_0 - sxz_mz) + b * (sxz_pz - sxz_m2z)) * s.rdz;
-  float vx = vxi + (s.dt * bxi) * (dsxx + dsxz_z);
-  const float dsxz_x = (a * (sxz_0 - sxz_m1) + b * (sxz_p1 - sxz_m2)) * s.rdx;
-  const float dszz = (a * (szz_pz - szz_0) + b * (szz_p2z - szz_mz)) * s.rdz;
-  float vz = vzc + (s.dt * bzi) * (dsxz_x + dszz);
-  vx = vx * g;
-  vz = vz * g;
+  const float dsxx = (a * (v.sxx_p1 - v.sxx_0) + b * (v.sxx_p2 - v.sxx_m1)) * s.rdx;
+  const float dsxz_z = (a * (v.sxz_0 - v.sxz_mz) + b * (v.sxz_pz - v.sxz_m2z)) * s.rdz;
+  float vx = v.vxi + (s.dt * v.bxi) * (dsxx + dsxz_z);
+  const float dsxz_x = (a * (v.sxz_0 - v.sxz_m1) + b * (v.sxz_p1 - v.sxz_m2)) * s.rdx;
+  const float dszz = (a * (v.szz_pz - v.szz_0) + b * (v.szz_p2z - v.szz_mz)) * s.rdz;
+  float vz = vzc + (s.dt * v.bzi) * (dsxz_x + dszz);
+  vx = vx * v.g;
+  vz = vz * v.g;
   as_global(s.vx)[i] = vx;
   as_global(s.vz)[i] = vz;
+}
+
+__device__ __forceinline__ void ela_vel_cell(const ElaStep& s, int ix, int iz) {
+  ela_vel_finish(s, ix, iz, ela_vel_load(s, ix, iz));
 }
 
 // Vertical strain rate (4th-order staggered D-z of vz) at (iz, ix); rows
@@ -136,13 +158,57 @@ __device__ __forceinline__ float ela_strain_rate(const ElaStep& s, int iz, int i
 }
 
 // Stress pass at one cell (wave2d.c:158-195), free surface on row 0
-// (wave2d.c:198-211) and receiver recording; reads through
-// global-address-space pointers.
-__device__ __forceinline__ void ela_str_cell(const ElaStep& s, int ix, int iz) {
+// (wave2d.c:198-211) and receiver recording, in the same two halves: the
+// interior update's reads (ela_str_load; the pass writes only stresses and
+// receiver rows, which it does not read), then the rest (ela_str_finish).
+struct ElaStrIn {
+  bool upd;  // interior: the cell is updated
+  float vx_0, vx_m1, vx_p1, vx_m2, vx_pz, vx_p2z, vx_mz;
+  float vz_0, vz_mz, vz_pz, vz_m2z, vz_p1, vz_p2, vz_m1;
+  float l, m, sxx, szz, sxz, g;
+};
+
+__device__ __forceinline__ ElaStrIn ela_str_load(const ElaStep& s, int ix, int iz) {
+  const int nx = s.nx;
+  const int z0 = s.fs ? 1 : 2;
+  ElaStrIn v;
+  v.upd = !(iz == 0 || ix < 2 || ix >= nx - 2 || iz < z0 || iz >= s.nz - 2);
+  v.vx_0 = v.vx_m1 = v.vx_p1 = v.vx_m2 = v.vx_pz = v.vx_p2z = v.vx_mz = 0.0f;
+  v.vz_0 = v.vz_mz = v.vz_pz = v.vz_m2z = v.vz_p1 = v.vz_p2 = v.vz_m1 = 0.0f;
+  v.l = v.m = v.sxx = v.szz = v.sxz = 0.0f;
+  v.g = 1.0f;
+  if (v.upd) {
+    const int i = iz * nx + ix;
+    const gfloat* vx = as_global((const float*)s.vx);
+    const gfloat* vz = as_global((const float*)s.vz);
+    v.vx_0 = vx[i];
+    v.vx_m1 = vx[i - 1];
+    v.vx_p1 = vx[i + 1];
+    v.vx_m2 = vx[i - 2];
+    v.vx_pz = vx[i + nx];
+    v.vx_p2z = vx[i + 2 * nx];
+    v.vx_mz = vx[i - nx];
+    v.vz_0 = vz[i];
+    v.vz_mz = vz[i - nx];
+    v.vz_pz = vz[i + nx];
+    v.vz_m2z = i - 2 * nx < 0 ? 0.0f : vz[i - 2 * nx];  // rows above the grid read 0
+    v.vz_p1 = vz[i + 1];
+    v.vz_p2 = vz[i + 2];
+    v.vz_m1 = vz[i - 1];
+    v.l = as_global(s.lam)[i];
+    v.m = as_global(s.mu)[i];
+    v.sxx = as_global((const float*)s.sxx)[i];
+    v.szz = as_global((const float*)s.szz)[i];
+    v.sxz = as_global((const float*)s.sxz)[i];
+    v.g = ela_sponge(s, ix, iz, i);
+  }
+  return v;
+}
+
+__device__ __forceinline__ void ela_str_finish(const ElaStep& s, int ix, int iz,
+                                               const ElaStrIn& v) {
   const int nx = s.nx;
   const float a = TSW_S1, b = TSW_S2;
-  const gfloat* vx = as_global((const float*)s.vx);
-  const gfloat* vz = as_global((const float*)s.vz);
   if (s.rec_it >= 0 && iz == s.riz) {
     const int r0 = as_global(s.roff)[ix], r1 = as_global(s.roff)[ix + 1];
     if (r0 < r1) {
@@ -154,6 +220,7 @@ __device__ __forceinline__ void ela_str_cell(const ElaStep& s, int ix, int iz) {
   if (iz == 0) {
     if (!s.fs) return;
     // assignment without dt, as the reference does
+    const gfloat* vx = as_global((const float*)s.vx);
     as_global(s.szz)[ix] = 0.0f;
     as_global(s.sxz)[ix] = 0.0f;
     if (ix >= 2 && ix < nx - 2) {
@@ -163,30 +230,26 @@ __device__ __forceinline__ void ela_str_cell(const ElaStep& s, int ix, int iz) {
     }
     return;
   }
-  const int z0 = s.fs ? 1 : 2;
-  if (ix < 2 || ix >= nx - 2 || iz < z0 || iz >= s.nz - 2) return;
+  if (!v.upd) return;
   const int i = iz * nx + ix;
-  gfloat* sxxp = as_global(s.sxx);
-  gfloat* szzp = as_global(s.szz);
-  gfloat* sxzp = as_global(s.sxz);
-  const float dvx_x = (a * (vx[i] - vx[i - 1]) + b * (vx[i + 1] - vx[i - 2])) * s.rdx;
-  // rows above the grid read 0
-  const float vz_m2z = i - 2 * nx < 0 ? 0.0f : vz[i - 2 * nx];
-  const float dvz_z = (a * (vz[i] - vz[i - nx]) + b * (vz[i + nx] - vz_m2z)) * s.rdz;
-  const float l = as_global(s.lam)[i], m = as_global(s.mu)[i];
-  const float l2m = l + 2.0f * m;
-  float sxx = sxxp[i] + s.dt * (l2m * dvx_x + l * dvz_z);
-  float szz = szzp[i] + s.dt * (l * dvx_x + l2m * dvz_z);
-  const float dvz_x = (a * (vz[i + 1] - vz[i]) + b * (vz[i + 2] - vz[i - 1])) * s.rdx;
-  const float dvx_z = (a * (vx[i + nx] - vx[i]) + b * (vx[i + 2 * nx] - vx[i - nx])) * s.rdz;
-  float sxz = sxzp[i] + (s.dt * m) * (dvz_x + dvx_z);
-  const float g = ela_sponge(s, ix, iz, i);
-  sxx = sxx * g;
-  szz = szz * g;
-  sxz = sxz * g;
-  sxxp[i] = sxx;
-  szzp[i] = szz;
-  sxzp[i] = sxz;
+  const float dvx_x = (a * (v.vx_0 - v.vx_m1) + b * (v.vx_p1 - v.vx_m2)) * s.rdx;
+  const float dvz_z = (a * (v.vz_0 - v.vz_mz) + b * (v.vz_pz - v.vz_m2z)) * s.rdz;
+  const float l2m = v.l + 2.0f * v.m;
+  float sxx = v.sxx + s.dt * (l2m * dvx_x + v.l * dvz_z);
+  float szz = v.szz + s.dt * (v.l * dvx_x + l2m * dvz_z);
+  const float dvz_x = (a * (v.vz_p1 - v.vz_0) + b * (v.vz_p2 - v.vz_m1)) * s.rdx;
+  const float dvx_z = (a * (v.vx_pz - v.vx_0) + b * (v.vx_p2z - v.vx_mz)) * s.rdz;
+  float sxz = v.sxz + (s.dt * v.m) * (dvz_x + dvx_z);
+  sxx = sxx * v.g;
+  szz = szz * v.g;
+  sxz = sxz * v.g;
+  as_global(s.sxx)[i] = sxx;
+  as_global(s.szz)[i] = szz;
+  as_global(s.sxz)[i] = sxz;
+}
+
+__device__ __forceinline__ void ela_str_cell(const ElaStep& s, int ix, int iz) {
+  ela_str_finish(s, ix, iz, ela_str_load(s, ix, iz));
 }
 
 void launch_elastic_velocity(const ElaStep& s, hipStream_t st);
