@@ -180,6 +180,8 @@ def fwi_leg(args, log):
     op = make_op(torch.float32, d32)
     if args.fwi_blocking >= 0:
         op.set_blocking(args.fwi_blocking)
+    if args.fwi_adjoint_blocking >= 0:
+        op.set_adjoint_blocking(args.fwi_adjoint_blocking)
     v = torch.tensor(cfg.start_vel, dtype=torch.float32, device="cuda")
     log(f"fwi: data modelled, operator resident in {time.time() - t0:.1f}s")
     op.forward_loss(v)
@@ -200,7 +202,8 @@ def fwi_leg(args, log):
            "gcell_updates_per_s": round(3 * steps / el / 1e9, 2),
            "effective_gbs": round(ach, 1), "frac_of_hbm_peak": round(ach / HBM_PEAK_GBS, 4),
            "algorithmic_bytes_per_shot_cell_step": FWI_BYTES_PER_SHOT_CELL_STEP,
-           "blocking": args.fwi_blocking, "loss": loss}
+           "blocking": args.fwi_blocking, "adjoint_blocking": args.fwi_adjoint_blocking,
+           "loss": loss}
     log(f"fwi: gradient in {el:.3f}s")
     if args.fwi_check:
         out["check"] = fwi_check(cfg, geo, src, make_op, observed, op, g32, v, d32, log)
@@ -353,6 +356,9 @@ def main():
     ap.add_argument("--fwi-blocking", type=int, default=-1,
                     help="misfit gradient forward passes: -1 library default, 0 one launch per "
                          "step, c >= 1 temporally blocked configuration c")
+    ap.add_argument("--fwi-adjoint-blocking", type=int, default=-1,
+                    help="misfit gradient adjoint: -1 library default, 0 one launch per step, "
+                         "c >= 1 temporally blocked configuration c")
     ap.add_argument("--fwi-check", action="store_true",
                     help="FP64 finite-difference check of the gradient (about a minute)")
     ap.add_argument("--served-requests", type=int, default=400)

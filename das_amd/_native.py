@@ -68,6 +68,7 @@ SIGNATURES = {
     "tsw_fwi_forward": (I, [P, P, P, ctypes.POINTER(D)]),
     "tsw_fwi_backward": (I, [P, P, P, P]),
     "tsw_fwi_set_blocking": (I, [P, I]),
+    "tsw_fwi_set_adjoint_blocking": (I, [P, I]),
     "tsw_fwi_set_adjoint_steps": (I, [P, I]),
     "tsw_fwi_predicted": (I, [P, I, P]),
     "tsw_fwi_set_residual": (I, [P, I, P]),

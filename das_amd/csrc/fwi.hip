@@ -721,6 +721,214 @@ static const FwiTbCfg kFwiTb[] = {
 };
 constexpr int kFwiTbN = (int)(sizeof(kFwiTb) / sizeof(kFwiTb[0]));
 constexpr int kFwiDefaultTb = 3;
+constexpr int kFwiDefaultAdjTb = 2;
+
+// Temporally blocked FP32 adjoint: nsub adjoint steps n = it0, it0-1, ... of
+// every shot of a group in one launch, with the tiling of k_fwi_tb (2T halo).
+// Each thread keeps Lam^{n+1}, Lam^{n+2}, Lam^{n+3}, C, S and its g_C in
+// registers for its R rows; C.w^{n+1} moves through double-buffered LDS with
+// one barrier per step.  Per step and cell it evaluates the formulas of
+// k_fwi_adj in the same FP32 order (A^n, the probe, R^T r_{n-1}, the gradient
+// term w^{n+1}.L u^n from the recompute's segment), so the gradient and the
+// adjoint field are bit-identical to one launch per step.  The state at
+// block boundaries ping-pongs between two sets, as in the forward.
+struct FwiAdjTbArgs {
+  int nx, nz, fs, n_sponge;
+  float rdx2, rdz2;
+  const float* C;
+  const float* table;        // sponge factor by depth
+  long long n;               // cells per shot
+  const float* rd[3];        // set read: Lam^{n+1}, Lam^{n+2}, Lam^{n+3} (shot j at + j * n)
+  float* wr[3];              // set written: the same for the next block
+  const float* lu;           // segment of L u^it: lu + (it - lu_it0) * sstride + j * n
+  long long sstride;
+  int lu_it0;
+  float* grad;               // g_C (shot j at + j * n)
+  const FwiMeta* meta;
+  int it0, nsub;
+  int tiles_x, tiles;        // tiles per shot
+  int count;                 // shots: the grid is tiles * count workgroups
+};
+
+template <int T, int R, int WX, int WZ>
+__global__ __launch_bounds__(64 * WX * WZ)
+__attribute__((amdgpu_waves_per_eu(kTbWavesPerEu<R, WX, WZ>))) void k_fwi_adj_tb(FwiAdjTbArgs a) {
+  using S = TbShape<T, R, WX, WZ>;
+  __shared__ float lds[2][S::LR * S::LS];
+  const int b = xcd_tile_tb(blockIdx.x, a.tiles * a.count);
+  const int shot = b / a.tiles, t = b - shot * a.tiles;
+  const FwiMeta& m = a.meta[shot];
+  const long long o = (long long)shot * a.n;
+  const int nx = a.nx, nz = a.nz;
+  const int cx0 = (t % a.tiles_x) * S::W;
+  const int cz0 = (t / a.tiles_x) * S::H;
+  const int X0 = cx0 - 2 * T > 0 ? cx0 - 2 * T : 0;
+  const int Z0 = cz0 - 2 * T > 0 ? cz0 - 2 * T : 0;
+  const int lx = threadIdx.x, ly = threadIdx.y;
+  const int col = X0 + lx;
+  const int row0 = Z0 + ly * R;
+  const bool col_in = col < nx;
+  const bool core_col = col_in && col >= cx0 && col < cx0 + S::W;
+  const bool col_int = col >= 2 && col < nx - 2;
+  unsigned m_in = 0, m_int = 0, m_core = 0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int row = row0 + q;
+    if (col_in && row < nz) {
+      m_in |= 1u << q;
+      if (col_int && row >= 2 && row < nz - 2) m_int |= 1u << q;
+      if (core_col && row >= cz0 && row < cz0 + S::H) m_core |= 1u << q;
+    }
+  }
+  // rows 0 and 1 under a free surface (their Q^T Lam is 0); row 2 is the
+  // thread's row q = 2 exactly when row0 == 0, and its Q^T subtracts row 1
+  const bool top = a.fs && row0 == 0;
+  const unsigned m_out = m_in & ~m_int;  // in the grid, outside I
+  float l1[R], l2[R], l3[R], cf[R], sp[R], g[R], cw[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    l1[q] = l2[q] = l3[q] = cf[q] = g[q] = 0.0f;
+    sp[q] = 1.0f;
+    if (m_in >> q & 1u) {
+      const int idx = (row0 + q) * nx + col;
+      cf[q] = a.C[idx];
+      // table[0] = 1
+      sp[q] = a.table[sponge_depth(col, row0 + q, nx, nz, a.n_sponge, a.fs)];
+      l1[q] = a.rd[0][o + idx];
+      l2[q] = a.rd[1][o + idx];
+      if (m_out >> q & 1u) l3[q] = a.rd[2][o + idx];
+      if ((m_core & m_int) >> q & 1u) g[q] = a.grad[o + idx];
+    }
+  }
+  // C.w of a level on I (0 elsewhere): C * (S * zeta(Lam))
+  auto cw_of = [&](const float* lv, int q) -> float {
+    const float z = (top && q == 2) ? lv[2] - lv[1] : lv[q];
+    return (m_int >> q & 1u) ? cf[q] * (sp[q] * z) : 0.0f;
+  };
+  // C.w is 0 outside the grid, and cells on the grid's edge read it there
+  // (their A^n is L^T(C.w) + ...): the pad rows and columns of both LDS
+  // buffers, which no thread writes, are zeroed once
+  for (int i = ly * (64 * WX) + lx; i < 2 * S::LR * S::LS; i += S::THREADS) (&lds[0][0])[i] = 0.0f;
+  __syncthreads();
+  float* X = lds[0];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    cw[q] = cw_of(l1, q);
+    X[(ly * R + q + 2) * S::LS + lx + 2] = cw[q];
+  }
+  __syncthreads();
+  const float c0 = FwiK<float>::c0, c1 = fwi_c1<float>(), c2 = fwi_c2<float>();
+  const int riz = m.riz, nrec = m.nrec;
+  const bool rec_strip = riz >= row0 && riz < row0 + R && col_in;
+  int rec_r0 = 0, rec_r1 = 0;
+  if (rec_strip) {
+    rec_r0 = as_global(m.roff)[col];
+    rec_r1 = as_global(m.roff)[col + 1];
+  }
+  const int src_cell = m.src_cell;
+  for (int j = 0; j < a.nsub; ++j) {
+    const int it = a.it0 - j;
+    float* Y = lds[(j + 1) & 1];
+    const gfloat* lu = as_global(a.lu) + (long long)(it - a.lu_it0) * a.sstride + o;
+    float lun[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const unsigned idx = (unsigned)((row0 + q) * nx + col);
+      lun[q] = ((m_core & m_int) >> q & 1u) ? lu[idx] : 0.0f;
+    }
+    float nv[R];
+    const float* xr = X + 2;  // column offset of lx = 0
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int lr = (ly * R + q + 2) * S::LS + lx;
+      // L^T (C.w^{n+1}), evaluated on every lane
+      const float xm2 = xr[lr - 2], xm1 = xr[lr - 1], xp1 = xr[lr + 1], xp2 = xr[lr + 2];
+      const float zm2 = q >= 2 ? cw[q - 2] : xr[lr - 2 * S::LS];
+      const float zm1 = q >= 1 ? cw[q - 1] : xr[lr - S::LS];
+      const float zp1 = q + 1 < R ? cw[q + 1] : xr[lr + S::LS];
+      const float zp2 = q + 2 < R ? cw[q + 2] : xr[lr + 2 * S::LS];
+      float lxx = c2 * (xm2 + xp2);
+      lxx = lxx + c1 * (xm1 + xp1);
+      lxx = (lxx + c0 * cw[q]) * a.rdx2;
+      float lzz = c2 * (zm2 + zp2);
+      lzz = lzz + c1 * (zm1 + zp1);
+      lzz = (lzz + c0 * cw[q]) * a.rdz2;
+      const float lt = lxx + lzz;
+      float acc;
+      if (m_int >> q & 1u) {
+        const float z1 = (top && q == 2) ? l1[2] - l1[1] : l1[q];
+        const float z2 = (top && q == 2) ? l2[2] - l2[1] : l2[q];
+        const float w1 = sp[q] * z1;
+        const float w2 = sp[q] * z2;
+        acc = (2.0f * w1 + lt) - w2;
+        if (m_core >> q & 1u) g[q] = g[q] + w1 * lun[q];
+      } else if (m_in >> q & 1u) {
+        acc = lt;
+        if (!(top && q < 2)) acc = acc + ((top && q == 2) ? l3[2] - l3[1] : l3[q]);
+      } else {
+        acc = 0.0f;  // outside the grid
+      }
+      nv[q] = acc;
+    }
+    // probe: A^n at the source cell, before the receiver term
+    if (m_core) {
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+        if ((m_core >> q & 1u) && (row0 + q) * nx + col == src_cell)
+          as_global((float*)m.probe)[it] = nv[q];
+    }
+    // Lam^n = A^n + R^T r_{n-1}, receivers in order
+    if (it >= 1 && rec_r0 < rec_r1) {
+      const gfloat* row = as_global((const float*)m.res) + (long long)(it - 1) * nrec;
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+        if (row0 + q == riz)
+          for (int r = rec_r0; r < rec_r1; ++r) nv[q] = nv[q] + row[as_global(m.ridx)[r]];
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      l3[q] = l2[q];
+      l2[q] = l1[q];
+      l1[q] = nv[q];
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      cw[q] = cw_of(l1, q);
+      Y[(ly * R + q + 2) * S::LS + lx + 2] = cw[q];
+    }
+    X = Y;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    if (m_core >> q & 1u) {
+      const int idx = (row0 + q) * nx + col;
+      a.wr[0][o + idx] = l1[q];
+      a.wr[1][o + idx] = l2[q];
+      a.wr[2][o + idx] = l3[q];
+      if (m_int >> q & 1u) a.grad[o + idx] = g[q];
+    }
+  }
+}
+
+struct FwiAdjTbCfg {
+  int T, R, WX, WZ;
+  void (*launch)(const FwiAdjTbArgs&, hipStream_t);
+};
+template <int T, int R, int WX, int WZ>
+static void fwi_adj_tb_launch(const FwiAdjTbArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL((k_fwi_adj_tb<T, R, WX, WZ>), dim3(a.tiles * a.count), dim3(64 * WX, WZ),
+                     0, st, a);
+}
+// the shapes of kFwiTb, in the same order
+static const FwiAdjTbCfg kFwiAdjTb[] = {
+    {4, 4, 1, 16, fwi_adj_tb_launch<4, 4, 1, 16>}, {5, 4, 1, 16, fwi_adj_tb_launch<5, 4, 1, 16>},
+    {4, 8, 1, 8, fwi_adj_tb_launch<4, 8, 1, 8>},   {4, 16, 1, 4, fwi_adj_tb_launch<4, 16, 1, 4>},
+    {3, 4, 1, 16, fwi_adj_tb_launch<3, 4, 1, 16>}, {5, 8, 1, 8, fwi_adj_tb_launch<5, 8, 1, 8>},
+    {3, 8, 1, 8, fwi_adj_tb_launch<3, 8, 1, 8>},   {4, 12, 1, 8, fwi_adj_tb_launch<4, 12, 1, 8>},
+    {6, 8, 1, 8, fwi_adj_tb_launch<6, 8, 1, 8>},
+};
+static_assert(sizeof(kFwiAdjTb) / sizeof(kFwiAdjTb[0]) == kFwiTbN, "one adjoint shape per forward shape");
 
 struct FwiShotHost {
   int nt, six, siz, riz, nrec;
@@ -744,7 +952,9 @@ struct Fwi {
   DevBuf S, C, buf, mu, seg, gC, partial, loss;
   int tb = 0;          // FP32 blocked forward: configuration kFwiTb[tb - 1]; 0 = per step
   int adj_steps = 1;   // adjoint steps per launch (1 or 2; 2 measured slower on MI355X)
+  int atb = 0;         // FP32 blocked adjoint: configuration kFwiAdjTb[atb - 1]; 0 = per step
   DevBuf sets, table;  // blocked forward: two state sets [2][3][cmax][n], sponge table
+  DevBuf asets;        // blocked adjoint: two state sets [2][3][cmax][n]
   double last_loss = 0;
   bool have_forward = false;
   size_t esz() const { return prec == 8 ? 8 : 4; }
@@ -936,6 +1146,50 @@ static int fwi_forward_t(Fwi* F, const T* d_vel, hipStream_t st, bool store_all)
   return kOk;
 }
 
+static float* atb_set(const Fwi* F, int k, int l) {
+  return (float*)F->asets.p + (long long)(k * 3 + l) * F->cmax * F->cells();
+}
+
+// Blocked adjoint steps n1-1 down to n0 of a group from state set *k
+// (updated); `lu` holds L u^it of the segment starting at n0.
+static void adj_tb_run(const Fwi* F, const FwiGroup& G, int n0, int n1, int* k, const float* lu,
+                       hipStream_t st) {
+  const FwiAdjTbCfg& c = kFwiAdjTb[F->atb - 1];
+  FwiAdjTbArgs a;
+  std::memset(&a, 0, sizeof a);
+  const float dxf = (float)F->dx, dzf = (float)F->dz;
+  a.nx = F->nx;
+  a.nz = F->nz;
+  a.fs = F->fs;
+  a.n_sponge = F->n_sponge;
+  a.rdx2 = 1.0f / (dxf * dxf);
+  a.rdz2 = 1.0f / (dzf * dzf);
+  a.C = (const float*)F->C.p;
+  a.table = F->table.f();
+  a.n = F->cells();
+  a.lu = lu;
+  a.sstride = (long long)G.count() * F->cells();
+  a.lu_it0 = n0;
+  a.grad = (float*)F->gC.p;
+  a.meta = (const FwiMeta*)G.meta.p;
+  a.count = G.count();
+  const int W = 64 * c.WX - 4 * c.T, H = c.WZ * c.R - 4 * c.T;
+  a.tiles_x = (F->nx + W - 1) / W;
+  a.tiles = a.tiles_x * ((F->nz + H - 1) / H);
+  for (int it = n1 - 1; it >= n0;) {
+    const int len = std::min(c.T, it - n0 + 1);
+    for (int l = 0; l < 3; ++l) {
+      a.rd[l] = atb_set(F, *k, l);
+      a.wr[l] = atb_set(F, *k ^ 1, l);
+    }
+    a.it0 = it;
+    a.nsub = len;
+    c.launch(a, st);
+    *k ^= 1;
+    it -= len;
+  }
+}
+
 template <typename T>
 static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, bool store_all) {
   const long long n = F->cells();
@@ -946,11 +1200,23 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
   for (auto& G : F->groups) {
     const int nt = G.nt, cnt = G.count();
     const long long cn = (long long)cnt * n;
-    for (int b = 0; b < kAdjRing; ++b)
-      hipLaunchKernelGGL(k_fwi_fill<T>, dim3(g1(cn)), dim3(256), 0, st, adj<T>(F, b), cn, T(0));
-    // Lam^{nt} = R^T r_{nt-1}; Lam^{nt+1..nt+3} = 0
-    hipLaunchKernelGGL(k_fwi_seed<T>, dim3((F->nx + 255) / 256, cnt), dim3(256), 0, st,
-                       adj<T>(F, nt), n, F->nx, (const FwiMeta*)G.meta.p, nt - 1);
+    // Lam^{nt} = R^T r_{nt-1}; Lam^{nt+1..nt+3} = 0: in the ring of the per-step
+    // kernels, or in state set 0 (Lam^{n+1}, Lam^{n+2}, Lam^{n+3}) of the blocked one
+    const bool blocked_adj = std::is_same<T, float>::value && F->atb > 0;
+    int ak = 0;
+    if (blocked_adj) {
+      for (int l = 0; l < 3; ++l)
+        hipLaunchKernelGGL(k_fwi_fill<float>, dim3(g1(cn)), dim3(256), 0, st, atb_set(F, 0, l),
+                           cn, 0.0f);
+      hipLaunchKernelGGL(k_fwi_seed<float>, dim3((F->nx + 255) / 256, cnt), dim3(256), 0, st,
+                         atb_set(F, 0, 0), n, F->nx, (const FwiMeta*)G.meta.p, nt - 1);
+    } else {
+      for (int b = 0; b < kAdjRing; ++b)
+        hipLaunchKernelGGL(k_fwi_fill<T>, dim3(g1(cn)), dim3(256), 0, st, adj<T>(F, b), cn,
+                           T(0));
+      hipLaunchKernelGGL(k_fwi_seed<T>, dim3((F->nx + 255) / 256, cnt), dim3(256), 0, st,
+                         adj<T>(F, nt), n, F->nx, (const FwiMeta*)G.meta.p, nt - 1);
+    }
     AdjArgs<T> a;
     std::memset(&a, 0, sizeof a);
     const T dxT = (T)F->dx, dzT = (T)F->dz;
@@ -999,6 +1265,10 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
       }
       for (int it = n0; it < n1; ++it)
         fwd_step<T>(F, G, s, it, (T*)F->seg.p + (long long)(it - n0) * cn, false, st);
+      }
+      if (blocked_adj) {
+        adj_tb_run(F, G, n0, n1, &ak, (const float*)F->seg.p, st);
+        continue;
       }
       for (int it = n1 - 1; it >= n0;) {
         a.lam0 = adj<T>(F, it);
@@ -1164,11 +1434,13 @@ int tsw_fwi_create(int nx, int nz, double dx, double dz, double dt, int n_sponge
   if (precision == 4) {  // blocked forward by default in FP32
     std::vector<float> tab;
     sponge_table(tab, n_sponge, (float)sponge_alpha);
-    if ((rc = F->sets.ensure(6 * cn)) || (rc = F->table.ensure(tab.size() * sizeof(float))))
+    if ((rc = F->sets.ensure(6 * cn)) || (rc = F->asets.ensure(6 * cn)) ||
+        (rc = F->table.ensure(tab.size() * sizeof(float))))
       return fail(rc);
     if (hipMemcpy(F->table.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice))
       return fail((set_error("sponge table upload failed"), kErrHip));
     F->tb = kFwiDefaultTb;
+    F->atb = kFwiDefaultAdjTb;
   }
   *out = reinterpret_cast<tsw_fwi_t>(F);
   return kOk;
@@ -1206,6 +1478,16 @@ int tsw_fwi_set_blocking(tsw_fwi_t h, int config) {
   TSW_ARG(config == 0 || F->prec == 4, "the blocked forward is FP32 only");
   F->tb = config;
   F->have_forward = false;  // checkpoints of another path: run the forward again
+  return kOk;
+}
+
+int tsw_fwi_set_adjoint_blocking(tsw_fwi_t h, int config) {
+  Fwi* F = reinterpret_cast<Fwi*>(h);
+  TSW_ARG(F, "null argument");
+  TSW_ARG(config >= 0 && config <= kFwiTbN, "adjoint blocking configuration %d outside [0, %d]",
+          config, kFwiTbN);
+  TSW_ARG(config == 0 || F->prec == 4, "the blocked adjoint is FP32 only");
+  F->atb = config;
   return kOk;
 }
 

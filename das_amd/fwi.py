@@ -126,16 +126,23 @@ class WaveformMisfit:
         return g
 
     def set_blocking(self, config: int):
-        """FP32 forward and recompute: 0 = one launch per step, 1..6 = a
+        """FP32 forward and recompute: 0 = one launch per step, 1..9 = a
         temporally blocked configuration (default 3); results are
         bit-identical either way."""
         N.check(N.lib().tsw_fwi_set_blocking(self._h, int(config)))
         self._last_vel = None
 
+    def set_adjoint_blocking(self, config: int):
+        """FP32 adjoint: 0 = one launch per step (or two, see
+        set_adjoint_steps), 1..9 = a temporally blocked configuration, the
+        shapes of set_blocking (default 2); the gradient is bit-identical
+        either way."""
+        N.check(N.lib().tsw_fwi_set_adjoint_blocking(self._h, int(config)))
+
     def set_adjoint_steps(self, steps: int):
-        """Adjoint steps per launch: 1 (default), or 2 (one launch computes two
-        consecutive levels; slower on MI355X at 300 x 800).  The gradient is
-        bit-identical either way."""
+        """Per-step adjoint (set_adjoint_blocking(0)): steps per launch, 1
+        (default), or 2 (one launch computes two consecutive levels; slower on
+        MI355X at 300 x 800).  The gradient is bit-identical either way."""
         N.check(N.lib().tsw_fwi_set_adjoint_steps(self._h, int(steps)))
 
     def predicted(self, shot: int) -> np.ndarray:

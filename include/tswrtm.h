@@ -265,10 +265,15 @@ int tsw_fwi_create(int nx, int nz, double dx, double dz, double dt, int n_sponge
 int tsw_fwi_forward(tsw_fwi_t fwi, const void* d_vel, void* stream, double* loss_out);
 /* dJ/dv into d_grad (adjoint sweep with recompute); needs tsw_fwi_forward first. */
 int tsw_fwi_backward(tsw_fwi_t fwi, const void* d_vel, void* d_grad, void* stream);
-/* FP32 forward and recompute: 0 = one launch per step, c = 1..6 = temporally
- * blocked configuration c = 1..9 (the default is 3).  FP64 runs one launch per step. */
+/* FP32 forward and recompute: 0 = one launch per step, c = 1..9 = temporally
+ * blocked configuration c (the default is 3).  FP64 runs one launch per step. */
 int tsw_fwi_set_blocking(tsw_fwi_t fwi, int config);
-/* Adjoint steps per launch, 1 (default) or 2; the same bits either way. */
+/* FP32 adjoint: 0 = the per-step kernels (see tsw_fwi_set_adjoint_steps),
+ * c = 1..9 = temporally blocked configuration c, the shapes of
+ * tsw_fwi_set_blocking (the default is 2).  The same bits either way.  FP64
+ * runs the per-step kernels. */
+int tsw_fwi_set_adjoint_blocking(tsw_fwi_t fwi, int config);
+/* Per-step adjoint: steps per launch, 1 (default) or 2; the same bits either way. */
 int tsw_fwi_set_adjoint_steps(tsw_fwi_t fwi, int steps);
 /* Predicted data of a shot from the last forward, (nt, nrec), to the host. */
 int tsw_fwi_predicted(tsw_fwi_t fwi, int shot, void* host_out);

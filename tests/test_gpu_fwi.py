@@ -332,9 +332,51 @@ def test_two_step_adjoint_equals_single_steps(dtype):
         out = []
         for steps in (1, 2):
             op = _op(shots, dtype, ckpt=ck)
+            if dtype == np.float32:
+                op.set_adjoint_blocking(0)  # the per-step kernels
             op.set_adjoint_steps(steps)
             g = op.gradient(v).cpu().numpy()
             out.append((g, op.adjoint_at_source(0), op.adjoint_at_source(1)))
             op.close()
         for a, b in zip(*out):
             np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("fs", [True, False])
+@pytest.mark.parametrize("config", [1, 2, 3, 4, 5, 6, 7, 8, 9])
+def test_blocked_adjoint_equals_per_step(config, fs):
+    """Temporally blocked FP32 adjoint: gradient and adjoint at the sources
+    bit-identical to one launch per step, with checkpoints of several lengths
+    and with every level kept, for shots of two lengths whose sources and
+    receivers sit on the free-surface rows and in never-updated edge cells."""
+    from das_amd.fwi import MisfitShot
+    from das_amd.wave_solver import ricker_wavelet
+
+    v_true, v0 = _models()
+    src, _ = ricker_wavelet(30.0, DT, NT * DT)
+    src = src.astype(np.float32)
+    geo = [(20, 2, np.arange(0, NX, 2), 2), (1, 1, np.arange(NX), 0),
+           (NX - 2, NZ - 3, np.arange(3, NX - 1, 5), 1), (40, 5, np.array([7, 7, 30]), 3)]
+    shots = [MisfitShot(src=src if i != 1 else src[:NT - 29], six=a, siz=b,
+                        rix=r.astype(np.int32), riz=z,
+                        d_obs=np.zeros((NT if i != 1 else NT - 29, len(r)), np.float32))
+             for i, (a, b, r, z) in enumerate(geo)]
+    op = _op(shots, np.float32, fs=fs)
+    op.forward_loss(_dev(v_true, np.float32))
+    for i, sh in enumerate(shots):
+        sh.d_obs = op.predicted(i)
+    op.close()
+    v = _dev(v0, np.float32)
+    for ck in (0, -1, 7):
+        out = []
+        for blk in (0, config):
+            op = _op(shots, np.float32, ckpt=ck, fs=fs)
+            op.set_adjoint_blocking(blk)
+            g = op.gradient(v).cpu().numpy()
+            out.append([g] + [op.adjoint_at_source(i) for i in range(len(shots))])
+            op.close()
+        assert np.abs(out[0][0]).max() > 0
+        for a, b in zip(*out):
+            np.testing.assert_array_equal(a, b)
+    with pytest.raises(ValueError):
+        _op(shots[:1], np.float64).set_adjoint_blocking(1)
