@@ -574,7 +574,8 @@ struct FwiTbArgs {
 };
 
 template <int T, int R, int WX, int WZ>
-__global__ __launch_bounds__(64 * WX * WZ) void k_fwi_tb(FwiTbArgs a) {
+__global__ __launch_bounds__(64 * WX * WZ)
+__attribute__((amdgpu_waves_per_eu(kTbWavesPerEu<R, WX, WZ>))) void k_fwi_tb(FwiTbArgs a) {
   using S = TbShape<T, R, WX, WZ>;
   __shared__ float lds[2][S::LR * S::LS];
   const int b = xcd_tile_tb(blockIdx.x, a.tiles * a.count);
@@ -720,7 +721,7 @@ static const FwiTbCfg kFwiTb[] = {
     {6, 8, 1, 8, fwi_tb_launch<6, 8, 1, 8>},
 };
 constexpr int kFwiTbN = (int)(sizeof(kFwiTb) / sizeof(kFwiTb[0]));
-constexpr int kFwiDefaultTb = 3;
+constexpr int kFwiDefaultTb = 2;
 constexpr int kFwiDefaultAdjTb = 2;
 
 // Temporally blocked FP32 adjoint: nsub adjoint steps n = it0, it0-1, ... of

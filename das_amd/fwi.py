@@ -127,7 +127,7 @@ class WaveformMisfit:
 
     def set_blocking(self, config: int):
         """FP32 forward and recompute: 0 = one launch per step, 1..9 = a
-        temporally blocked configuration (default 3); results are
+        temporally blocked configuration (default 2); results are
         bit-identical either way."""
         N.check(N.lib().tsw_fwi_set_blocking(self._h, int(config)))
         self._last_vel = None

@@ -266,7 +266,7 @@ int tsw_fwi_forward(tsw_fwi_t fwi, const void* d_vel, void* stream, double* loss
 /* dJ/dv into d_grad (adjoint sweep with recompute); needs tsw_fwi_forward first. */
 int tsw_fwi_backward(tsw_fwi_t fwi, const void* d_vel, void* d_grad, void* stream);
 /* FP32 forward and recompute: 0 = one launch per step, c = 1..9 = temporally
- * blocked configuration c (the default is 3).  FP64 runs one launch per step. */
+ * blocked configuration c (the default is 2).  FP64 runs one launch per step. */
 int tsw_fwi_set_blocking(tsw_fwi_t fwi, int config);
 /* FP32 adjoint: 0 = the per-step kernels (see tsw_fwi_set_adjoint_steps),
  * c = 1..9 = temporally blocked configuration c, the shapes of
