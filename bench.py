@@ -126,6 +126,10 @@ def served_leg(args, acoustic_gcell, elastic_gcell, kernel, log):
                     log=log)
     srv.close()
     n_el = sum(s.kind == "elastic" for s in cfg.specs)
+    # algorithmic HBM traffic of the stream: 26 B per acoustic RTM cell-update
+    # (k=2), 60 B per elastic cell-update, over the span of the run
+    traffic = sum(2 * s.nt * s.cells * (ELASTIC_BYTES_PER_CELL_UPDATE if s.kind == "elastic"
+                                        else BYTES_PER_CELL_UPDATE) for s in cfg.specs)
     return {"workload": f"{len(cfg.specs)} Poisson-arrival virtual-shot migrations, "
                         f"{len(cfg.specs) - n_el} acoustic / {n_el} elastic, nz {{256,384,512}} "
                         "x nx {512,1024,2048}, nt 2000-8000, dx 5 m, dt 0.4 ms, 200 GB budget, "
@@ -136,6 +140,8 @@ def served_leg(args, acoustic_gcell, elastic_gcell, kernel, log):
             "p99_latency_s": round(r["p99_latency_s"], 4),
             "shots_per_s": round(r["shots_per_s"], 3),
             "gcell_updates_per_s": round(r["gcell_updates_per_s"], 2),
+            "hbm_gbs_algorithmic": (round(traffic / r["span_s"] / 1e9, 1)
+                                    if r["failed"] == 0 and r["span_s"] else None),
             "failed": r["failed"], "span_s": round(r["span_s"], 2),
             "gpu_busy_s": round(r["gpu_busy_s"], 2), "max_active": args.served_slots,
             "kernel": kernel}
