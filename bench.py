@@ -210,6 +210,16 @@ def fwi_leg(args, log):
            "algorithmic_bytes_per_shot_cell_step": FWI_BYTES_PER_SHOT_CELL_STEP,
            "blocking": args.fwi_blocking, "adjoint_blocking": args.fwi_adjoint_blocking,
            "loss": loss}
+    tp = os.path.join(ROOT, "profiles", "gradient_kernels_hbm_traffic.json")
+    if os.path.exists(tp) and args.fwi_blocking < 0 and args.fwi_adjoint_blocking < 0:
+        # HBM bytes the default kernels move per shot-cell-step (rocprofv3 FETCH_SIZE +
+        # WRITE_SIZE on a shortened run of this workload), at this run's time
+        with open(tp) as f:
+            per = json.load(f)["gradient_total_B_per_shot_cell_step"]
+        out["traffic"] = {"bytes_per_shot_cell_step": per,
+                          "gbs": round(per * steps / el / 1e9, 1),
+                          "frac_of_hbm_peak": round(per * steps / el / 1e9 / HBM_PEAK_GBS, 4),
+                          "source": "profiles/gradient_kernels_hbm_traffic.json"}
     log(f"fwi: gradient in {el:.3f}s")
     if args.fwi_check:
         out["check"] = fwi_check(cfg, geo, src, make_op, observed, op, g32, v, d32, log)
