@@ -12,8 +12,10 @@
 // The FFTs are hipFFT library transforms (plain, batched).
 #include <hipfft/hipfft.h>
 
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "engine.h"
@@ -22,6 +24,7 @@
 namespace tsw {
 
 constexpr int kMaxSections = 16;
+constexpr int kSosChunk = 16;  // samples read ahead of the recursion
 
 struct SosArgs {
   const float* x;   // (nt, nch) row-major
@@ -44,45 +47,78 @@ __device__ __forceinline__ float sos_ext(const SosArgs& a, int c, int e) {
 }
 
 // One thread per channel: consecutive channels are consecutive addresses.
+// NS > 0: exactly NS sections, unrolled, with the state and coefficients in
+// registers; NS == 0: any count up to kMaxSections (state in private memory).
+template <int NS>
 __global__ void k_sosfiltfilt(SosArgs a) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.nch) return;
+  constexpr int MS = NS > 0 ? NS : kMaxSections;
+  const int nsec = NS > 0 ? NS : a.nsec;
   const int ne = a.nt + 2 * a.padlen;
-  double z[kMaxSections][2];
+  double z[MS][2], b0[MS], b1[MS], b2[MS], a1[MS], a2[MS];
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    b0[s] = a.sos[s][0];
+    b1[s] = a.sos[s][1];
+    b2[s] = a.sos[s][2];
+    a1[s] = a.sos[s][4];
+    a2[s] = a.sos[s][5];
+  }
   const double x0 = (double)sos_ext(a, c, 0);
-  for (int s = 0; s < a.nsec; ++s) {
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
     z[s][0] = a.zi[s][0] * x0;
     z[s][1] = a.zi[s][1] * x0;
   }
-  double y = 0.0;
-  for (int e = 0; e < ne; ++e) {
-    double xn = (double)sos_ext(a, c, e);
-    for (int s = 0; s < a.nsec; ++s) {
+  // one sample through the cascade, in scipy's (direct form II transposed) order
+  auto cascade = [&](double xn) -> double {
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+      if (NS == 0 && s >= nsec) break;
       const double xc = xn;
-      xn = a.sos[s][0] * xc + z[s][0];
-      z[s][0] = a.sos[s][1] * xc - a.sos[s][4] * xn + z[s][1];
-      z[s][1] = a.sos[s][2] * xc - a.sos[s][5] * xn;
+      xn = b0[s] * xc + z[s][0];
+      z[s][0] = b1[s] * xc - a1[s] * xn + z[s][1];
+      z[s][1] = b2[s] * xc - a2[s] * xn;
     }
-    y = xn;
-    a.tmp[(long long)e * a.nch + c] = y;
+    return xn;
+  };
+  // The recursion is sequential in time; the samples of each chunk of kSosChunk
+  // are read before the chunk's recursion so that one memory latency covers
+  // the chunk instead of one per sample.
+  double y = 0.0;
+  for (int e0 = 0; e0 < ne; e0 += kSosChunk) {
+    float xs[kSosChunk];
+#pragma unroll
+    for (int u = 0; u < kSosChunk; ++u) xs[u] = e0 + u < ne ? sos_ext(a, c, e0 + u) : 0.0f;
+#pragma unroll
+    for (int u = 0; u < kSosChunk; ++u) {
+      if (e0 + u >= ne) break;
+      y = cascade((double)xs[u]);
+      a.tmp[(long long)(e0 + u) * a.nch + c] = y;
+    }
   }
-  for (int s = 0; s < a.nsec; ++s) {
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
     z[s][0] = a.zi[s][0] * y;
     z[s][1] = a.zi[s][1] * y;
   }
-  for (int e = ne - 1; e >= 0; --e) {
-    double xn = a.tmp[(long long)e * a.nch + c];
-    for (int s = 0; s < a.nsec; ++s) {
-      const double xc = xn;
-      xn = a.sos[s][0] * xc + z[s][0];
-      z[s][0] = a.sos[s][1] * xc - a.sos[s][4] * xn + z[s][1];
-      z[s][1] = a.sos[s][2] * xc - a.sos[s][5] * xn;
-    }
-    const int t = e - a.padlen;
-    if (t >= 0 && t < a.nt && t % a.step == 0) {
-      float v = (float)xn;
-      if (a.sign) v = v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f);
-      a.out[(long long)(t / a.step) * a.nch + c] = v;
+  for (int e1 = ne - 1; e1 >= 0; e1 -= kSosChunk) {
+    double ts[kSosChunk];
+#pragma unroll
+    for (int u = 0; u < kSosChunk; ++u)
+      ts[u] = e1 - u >= 0 ? a.tmp[(long long)(e1 - u) * a.nch + c] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kSosChunk; ++u) {
+      const int e = e1 - u;
+      if (e < 0) break;
+      const double xn = cascade(ts[u]);
+      const int t = e - a.padlen;
+      if (t >= 0 && t < a.nt && t % a.step == 0) {
+        float v = (float)xn;
+        if (a.sign) v = v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f);
+        a.out[(long long)(t / a.step) * a.nch + c] = v;
+      }
     }
   }
 }
@@ -175,13 +211,50 @@ static unsigned gridn(long long n) {
     }                                                                              \
   } while (0)
 
-struct FftPlan {
-  hipfftHandle h = 0;
-  bool made = false;
-  ~FftPlan() {
-    if (made) (void)hipfftDestroy(h);
-  }
+// FFT plans and device buffers of the host-pointer DSP entry points, kept for
+// the life of the process: repeated calls of one shape (a block after block of
+// the same recording) make their plans once -- making a plan for a length such
+// as 30000 costs far more than running it -- and the buffers grow to the
+// largest call.  Every entry point holds the library mutex, which guards them.
+struct DspCache {
+  std::map<std::array<int, 3>, hipfftHandle> plans;  // (kind, n0, n1) -> plan
+  DevBuf buf[3];
 };
+
+static DspCache& dsp_cache() {
+  static DspCache* c = new DspCache();  // not destroyed: released with the process
+  return *c;
+}
+
+// kind 0: nch real-to-complex transforms of length n0 = nt along axis 0 of an
+// (nt, nch) array (n1 = nch); kind 1: the inverse (complex-to-real); kind 2: a
+// 2D complex transform of an (n0, n1) array.
+static int cached_plan(int kind, int n0, int n1, hipfftHandle* out) {
+  auto& plans = dsp_cache().plans;
+  const std::array<int, 3> key{kind, n0, n1};
+  auto it = plans.find(key);
+  if (it != plans.end()) {
+    *out = it->second;
+    return kOk;
+  }
+  hipfftHandle h = 0;
+  if (kind == 2) {
+    TSW_FFT(hipfftPlan2d(&h, n0, n1, HIPFFT_C2C));
+  } else {
+    const int nf = n0 / 2 + 1;
+    int n[1] = {n0}, re[1] = {n0}, cx[1] = {nf};
+    if (kind == 0) {
+      TSW_FFT(hipfftPlanMany(&h, 1, n, re, n1, 1, cx, n1, 1, HIPFFT_R2C, n1));
+    } else {
+      TSW_FFT(hipfftPlanMany(&h, 1, n, cx, n1, 1, re, n1, 1, HIPFFT_C2R, n1));
+    }
+  }
+  plans.emplace(key, h);
+  *out = h;
+  return kOk;
+}
+
+static DevBuf& scratch(int i) { return dsp_cache().buf[i]; }
 
 }  // namespace tsw
 
@@ -215,7 +288,7 @@ int tsw_sosfiltfilt(const float* x, int nt, int nch, const double* sos, int nsec
   a.step = step;
   a.sign = sign ? 1 : 0;
   const int nout = (nt + step - 1) / step;
-  DevBuf dx, dtmp, dout;
+  DevBuf &dx = scratch(0), &dtmp = scratch(1), &dout = scratch(2);
   TSW_TRY(dx.ensure((size_t)nt * nch * sizeof(float)));
   TSW_TRY(dtmp.ensure((size_t)(nt + 2 * padlen) * nch * sizeof(double)));
   TSW_TRY(dout.ensure((size_t)nout * nch * sizeof(float)));
@@ -223,7 +296,17 @@ int tsw_sosfiltfilt(const float* x, int nt, int nch, const double* sos, int nsec
   a.x = dx.f();
   a.tmp = (double*)dtmp.p;
   a.out = dout.f();
-  hipLaunchKernelGGL(k_sosfiltfilt, dim3((nch + 63) / 64), dim3(64), 0, st, a);
+  const dim3 grid((nch + 63) / 64), block(64);
+  switch (nsec) {  // the usual cascades (order 2 to 16) unrolled
+    case 1: hipLaunchKernelGGL(k_sosfiltfilt<1>, grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_sosfiltfilt<2>, grid, block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL(k_sosfiltfilt<3>, grid, block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_sosfiltfilt<4>, grid, block, 0, st, a); break;
+    case 5: hipLaunchKernelGGL(k_sosfiltfilt<5>, grid, block, 0, st, a); break;
+    case 6: hipLaunchKernelGGL(k_sosfiltfilt<6>, grid, block, 0, st, a); break;
+    case 8: hipLaunchKernelGGL(k_sosfiltfilt<8>, grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL(k_sosfiltfilt<0>, grid, block, 0, st, a); break;
+  }
   TSW_HIP(hipGetLastError());
   TSW_HIP(hipMemcpyAsync(out, dout.p, (size_t)nout * nch * sizeof(float), hipMemcpyDeviceToHost,
                          st));
@@ -238,7 +321,7 @@ int tsw_sign(const float* x, long long n, float* out) {
   Ctx* ctx;
   TSW_TRY(ctx_get(&ctx));
   hipStream_t st = ctx->stream;
-  DevBuf a, b;
+  DevBuf &a = scratch(0), &b = scratch(1);
   TSW_TRY(a.ensure((size_t)n * sizeof(float)));
   TSW_TRY(b.ensure((size_t)n * sizeof(float)));
   TSW_HIP(hipMemcpyAsync(a.p, x, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st));
@@ -261,24 +344,20 @@ int tsw_spectral_whitening(const float* x, int nt, int nch, double fs, double fm
   // bins k with fmin <= k fs / nt <= fmax
   const int k0 = std::max(0, (int)std::ceil(fmin * nt / fs - 1e-9));
   const int k1 = std::min(nf - 1, (int)std::floor(fmax * nt / fs + 1e-9));
-  DevBuf dx, dX;
+  DevBuf &dx = scratch(0), &dX = scratch(1);
   TSW_TRY(dx.ensure((size_t)nt * nch * sizeof(float)));
   TSW_TRY(dX.ensure((size_t)nf * nch * sizeof(hipfftComplex)));
   TSW_HIP(hipMemcpyAsync(dx.p, x, (size_t)nt * nch * sizeof(float), hipMemcpyHostToDevice, st));
-  int n[1] = {nt};
-  int inembed[1] = {nt}, onembed[1] = {nf};
-  FftPlan fwd, inv;
-  TSW_FFT(hipfftPlanMany(&fwd.h, 1, n, inembed, nch, 1, onembed, nch, 1, HIPFFT_R2C, nch));
-  fwd.made = true;
-  TSW_FFT(hipfftPlanMany(&inv.h, 1, n, onembed, nch, 1, inembed, nch, 1, HIPFFT_C2R, nch));
-  inv.made = true;
-  TSW_FFT(hipfftSetStream(fwd.h, st));
-  TSW_FFT(hipfftSetStream(inv.h, st));
-  TSW_FFT(hipfftExecR2C(fwd.h, dx.f(), (hipfftComplex*)dX.p));
+  hipfftHandle fwd, inv;
+  TSW_TRY(cached_plan(0, nt, nch, &fwd));
+  TSW_TRY(cached_plan(1, nt, nch, &inv));
+  TSW_FFT(hipfftSetStream(fwd, st));
+  TSW_FFT(hipfftSetStream(inv, st));
+  TSW_FFT(hipfftExecR2C(fwd, dx.f(), (hipfftComplex*)dX.p));
   hipLaunchKernelGGL(k_whiten, dim3(gridn((long long)nf * nch)), dim3(256), 0, st,
                      (hipfftComplex*)dX.p, nf, nch, k0, k1, (float)eps, (float)nt);
   TSW_HIP(hipGetLastError());
-  TSW_FFT(hipfftExecC2R(inv.h, (hipfftComplex*)dX.p, dx.f()));
+  TSW_FFT(hipfftExecC2R(inv, (hipfftComplex*)dX.p, dx.f()));
   TSW_HIP(hipMemcpyAsync(out, dx.p, (size_t)nt * nch * sizeof(float), hipMemcpyDeviceToHost, st));
   TSW_HIP(hipStreamSynchronize(st));
   return kOk;
@@ -292,21 +371,20 @@ int tsw_fk_filter(const float* g, int nt, int nx, double dt, double dx, double v
   TSW_TRY(ctx_get(&ctx));
   hipStream_t st = ctx->stream;
   const long long n = (long long)nt * nx;
-  DevBuf dg, dX;
+  DevBuf &dg = scratch(0), &dX = scratch(1);
   TSW_TRY(dg.ensure((size_t)n * sizeof(float)));
   TSW_TRY(dX.ensure((size_t)n * sizeof(hipfftComplex)));
   TSW_HIP(hipMemcpyAsync(dg.p, g, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_to_complex, dim3(gridn(n)), dim3(256), 0, st, dg.f(), n,
                      (hipfftComplex*)dX.p);
-  FftPlan p;
-  TSW_FFT(hipfftPlan2d(&p.h, nt, nx, HIPFFT_C2C));
-  p.made = true;
-  TSW_FFT(hipfftSetStream(p.h, st));
-  TSW_FFT(hipfftExecC2C(p.h, (hipfftComplex*)dX.p, (hipfftComplex*)dX.p, HIPFFT_FORWARD));
+  hipfftHandle p;
+  TSW_TRY(cached_plan(2, nt, nx, &p));
+  TSW_FFT(hipfftSetStream(p, st));
+  TSW_FFT(hipfftExecC2C(p, (hipfftComplex*)dX.p, (hipfftComplex*)dX.p, HIPFFT_FORWARD));
   hipLaunchKernelGGL(k_fk_mask, dim3(gridn(n)), dim3(256), 0, st, (hipfftComplex*)dX.p, nt, nx,
                      dt, dx, vmin, vmax, 1.0f / (float)n);
   TSW_HIP(hipGetLastError());
-  TSW_FFT(hipfftExecC2C(p.h, (hipfftComplex*)dX.p, (hipfftComplex*)dX.p, HIPFFT_BACKWARD));
+  TSW_FFT(hipfftExecC2C(p, (hipfftComplex*)dX.p, (hipfftComplex*)dX.p, HIPFFT_BACKWARD));
   hipLaunchKernelGGL(k_real_part, dim3(gridn(n)), dim3(256), 0, st, (const hipfftComplex*)dX.p,
                      n, dg.f());
   TSW_HIP(hipGetLastError());

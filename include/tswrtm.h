@@ -332,6 +332,9 @@ int tsw_save_matrix_csv(const char* path, const void* matrix, int precision, int
 
 /* Preprocessing of continuous data (run_das_pipeline.py:165-168, :280; the
  * reference's preprocessing module is absent, the steps follow README.md:27-52).
+ * These calls keep their FFT plans (per shape) and device scratch buffers (the
+ * largest call's size) until the process ends, so that block after block of
+ * one recording pays for neither again.
  *
  * Zero-phase filtering of x (nt, nch) along time with scipy.signal.sosfiltfilt
  * semantics: `sos` (nsec, 6) normalised sections, `zi` (nsec, 2) =
