@@ -76,3 +76,35 @@ def test_fk_filter_matches_numpy(nt, nx):
     e = np.linalg.norm(got - exact) / np.linalg.norm(exact)
     print(f"F-K rel L2 vs float64: {e:.2e}")
     assert e < 1e-5
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6, 7, 9, 12, 16, 18, 32])
+def test_zero_phase_filter_any_section_count(order):
+    """The filter kernel for every cascade length (1..8 sections unrolled, the
+    rest through the general kernel), low-pass and band-pass designs, equals
+    scipy.signal.sosfiltfilt bit for bit, as does its decimating form."""
+    from scipy import signal
+
+    from das_amd.preprocessing import _sosfiltfilt
+
+    x = _noise(1500, 5, seed=order, scale=3.0)
+    for sos in (signal.butter(order, 0.2, output="sos"),
+                signal.butter(max(1, order // 2), [0.05, 0.3], btype="band", output="sos")):
+        want = signal.sosfiltfilt(sos, x, axis=0).astype(np.float32)
+        np.testing.assert_array_equal(_sosfiltfilt(x, sos), want)
+        np.testing.assert_array_equal(_sosfiltfilt(x, sos, step=3), want[::3])
+
+
+def test_transform_plans_per_shape():
+    """Whitening and the F-K fan keep a plan per shape: shapes alternating
+    between calls give the same results as each shape alone."""
+    from das_amd.preprocessing import fk_filter, spectral_whitening
+
+    a, b = _noise(600, 7, seed=1), _noise(256, 12, seed=2)
+    wa, wb = spectral_whitening(a, 50.0, 1.0, 10.0), spectral_whitening(b, 50.0, 1.0, 10.0)
+    fa, fb = fk_filter(a, 0.02, 8.0, 100.0, 900.0), fk_filter(b, 0.02, 8.0, 100.0, 900.0)
+    for _ in range(2):
+        np.testing.assert_array_equal(spectral_whitening(b, 50.0, 1.0, 10.0), wb)
+        np.testing.assert_array_equal(spectral_whitening(a, 50.0, 1.0, 10.0), wa)
+        np.testing.assert_array_equal(fk_filter(b, 0.02, 8.0, 100.0, 900.0), fb)
+        np.testing.assert_array_equal(fk_filter(a, 0.02, 8.0, 100.0, 900.0), fa)
