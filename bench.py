@@ -97,7 +97,7 @@ def elastic_leg(args, log):
             "ms_per_shot": round(el * 1e3, 2),
             "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                         "kernel": "k_ela_batch (velocity + stress passes)", "launches": nl,
+                         "kernel": "k_ela_fused (velocity and stress passes, one launch per step)", "launches": nl,
                          "avg_launch_us": round(ms * 1e3 / max(nl, 1), 3),
                          "algorithmic_bytes_per_cell_update": ELASTIC_BYTES_PER_CELL_UPDATE}}
 

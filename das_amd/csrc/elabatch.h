@@ -9,8 +9,10 @@ namespace tsw {
 
 struct EShot {
   int nx, nz, nt, k, start, src_cell, riz, nrec;
-  float* f[5];          // forward vx, vz, sxx, szz, sxz
-  float* fb[5];         // back-propagation vx, vz, sxx, szz, sxz
+  float* f[5];          // forward vx, vz, sxx, szz, sxz: read by even steps
+  float* fb[5];         // back-propagation vx, vz, sxx, szz, sxz: read by even steps
+  float* f2[5];         // second forward set: read by odd steps
+  float* fb2[5];        // second back-propagation set: read by odd steps
   const float* mf[4];   // forward model: lambda, mu, bx, bz
   const float* mb[4];   // back-propagation model
   const float* src;
@@ -29,11 +31,12 @@ struct EConst {
   const float* table;
 };
 
-// One launch of the batched elastic step (pass 0 velocities, 1 stresses) over
+// One launch of the batched elastic step (velocity and stress passes) over
 // the shots in `list` (ids, then tile prefix) at global step g; shot time is
-// m = g - start (m < 0: initialisation).
-void ela_batch_launch(const EConst& c, const EShot* shots, const int* list, int nact,
-                      int ntiles, int g, int pass, hipStream_t st);
+// m = g - start (m < 0: initialisation).  Step m of a propagation reads the
+// set f/fb when m is even, f2/fb2 when odd, and writes the other.
+void ela_batch_step(const EConst& c, const EShot* shots, const int* list, int nact, int ntiles,
+                    int g, hipStream_t st);
 int ela_tiles(int nx, int nz, int* tiles_x);
 void ela_scale_launch(float* p, long long n, float w, hipStream_t st);
 

@@ -1,10 +1,11 @@
 // Elastic lane of a batch: many elastic shots of uneven size and length,
-// resident in HBM, advanced together.  Each global time step is two launches
-// (velocity pass, then stress pass) covering the tiles of every active elastic
-// shot; the steps are captured into hipGraphs and replayed on the lane's own
-// stream, so an elastic lane overlaps with the acoustic lane of the same batch.
-// The per-cell updates are the device functions of the single-shot elastic
-// path, so per-shot results are bit-identical to tsw_migrate_shot_elastic.
+// resident in HBM, advanced together.  Each global time step is one launch
+// (velocity and stress passes fused through LDS) covering the tiles of every
+// active elastic shot; the steps are captured into hipGraphs and replayed on
+// the lane's own stream, so an elastic lane overlaps with the acoustic lane of
+// the same batch.  The per-cell arithmetic is the device functions of the
+// single-shot elastic path, so per-shot results are bit-identical to
+// tsw_migrate_shot_elastic.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -15,11 +16,27 @@
 
 namespace tsw {
 
-constexpr int kEBX = 64;
-constexpr int kEBZ = 4;
-// rows per thread, kEBZ apart: a tile is kEBX x (kEBZ * kERows) cells
-constexpr int kERows = 2;
-constexpr int kETileZ = kEBZ * kERows;
+// One launch advances every active shot one full time step: the velocity
+// pass, then the stress pass, with the per-cell arithmetic of the two-pass
+// single-shot kernels.  A workgroup owns a core of kFW x kFH cells.  It stages
+// the old stresses of the core plus 4 cells on each side in LDS, computes the
+// new velocities of the core plus 2 cells on each side into LDS (the halo
+// velocities are recomputed by every tile that needs them; only core cells
+// are written out), then the new stresses of the core.  A step reads one
+// buffer set of a shot and writes the other, so no tile reads what another
+// tile of the same launch writes.
+constexpr int kFW = 56;            // core columns
+constexpr int kFC = kFW + 8;       // LDS columns: one wave per row
+constexpr int kFH = 8;             // core rows
+constexpr int kFWaves = 4;
+constexpr int kFSR = kFH + 8;      // stress rows staged
+constexpr int kFVR = kFH + 4;      // velocity rows computed
+constexpr int kFSQ = kFSR / kFWaves;
+constexpr int kFVQ = kFVR / kFWaves;
+constexpr int kFCQ = kFH / kFWaves;
+static_assert(kFC == 64, "one wave per LDS row");
+static_assert(kFSR % kFWaves == 0 && kFVR % kFWaves == 0 && kFH % kFWaves == 0,
+              "rows split evenly over the waves");
 
 __device__ __forceinline__ int xcd_tile_e(int b, int n) {
   const int per = n >> 3, rem = n & 7;
@@ -27,10 +44,11 @@ __device__ __forceinline__ int xcd_tile_e(int b, int n) {
   return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
 }
 
-// pass 0: velocities (with injection and side work), pass 1: stresses.
-__global__ __launch_bounds__(kEBX* kEBZ) void k_ela_batch(EConst c, const EShot* shots,
-                                                         const int* list, int nact, int ntiles,
-                                                         int g, int pass) {
+__global__ __launch_bounds__(kFC* kFWaves) void k_ela_fused(EConst c, const EShot* shots,
+                                                           const int* list, int nact, int ntiles,
+                                                           int g) {
+  __shared__ float s_xx[kFSR][kFC], s_zz[kFSR][kFC], s_xz[kFSR][kFC];
+  __shared__ float s_vx[kFVR][kFC], s_vz[kFVR][kFC];
   const int b = xcd_tile_e(blockIdx.x, ntiles);
   const int* pre = list + nact;
   int lo = 0, hi = nact - 1;
@@ -41,16 +59,18 @@ __global__ __launch_bounds__(kEBX* kEBZ) void k_ela_batch(EConst c, const EShot*
   }
   const EShot& S = shots[list[lo]];
   const int t = b - pre[lo];
-  const int ix = (t % S.tiles_x) * kEBX + threadIdx.x;
-  const int iz0 = (t / S.tiles_x) * kETileZ + threadIdx.y;
-  if (ix >= S.nx || iz0 >= S.nz) return;
+  // a wave is one threadIdx.y: its row index is wave-uniform
+  const int lane = threadIdx.x, wv = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int nx = S.nx, nz = S.nz;
+  const int x0 = (t % S.tiles_x) * kFW, z0 = (t / S.tiles_x) * kFH;
+  const int gx = x0 - 4 + lane;  // this lane's column
+  const bool xin = gx >= 0 && gx < nx;
+  const bool core_x = lane >= 4 && lane < 4 + kFW && gx < nx;
   const int m = g - S.start;
-  if (m < 0) {  // initialisation step of the shot
-    if (pass == 0) {
-      for (int r = 0; r < kERows; ++r) {
-        const int iz = iz0 + r * kEBZ;
-        if (iz >= S.nz) break;
-        const int i = iz * S.nx + ix;
+  if (m < 0) {  // initialisation step of the shot: the first read set, illum, image
+    if (core_x)
+      for (int r = wv; r < kFH && z0 + r < nz; r += kFWaves) {
+        const int i = (z0 + r) * nx + gx;
         for (int j = 0; j < 5; ++j) {
           S.f[j][i] = 0.0f;
           S.fb[j][i] = 0.0f;
@@ -58,95 +78,231 @@ __global__ __launch_bounds__(kEBX* kEBZ) void k_ela_batch(EConst c, const EShot*
         S.illum[i] = 0.0f;
         S.image[i] = 0.0f;
       }
-    }
     return;
   }
-  ElaStep s;
-  s.nx = S.nx;
-  s.nz = S.nz;
-  s.fs = c.fs;
-  s.rdx = c.rdx;
-  s.rdz = c.rdz;
-  s.dt = c.dt;
   const bool fwd = m < S.nt;
-  float* const* fld = fwd ? S.f : S.fb;
-  s.vx = fld[0];
-  s.vz = fld[1];
-  s.sxx = fld[2];
-  s.szz = fld[3];
-  s.sxz = fld[4];
-  s.sponge = nullptr;
-  s.sponge_table = c.table;
-  s.n_sponge = c.n_sponge;
-  s.src_cell = S.src_cell;
-  s.src = S.src;
-  s.src_it = -1;
-  s.riz = S.riz;
-  s.nrec = S.nrec;
-  s.roff = S.roff;
-  s.ridx = S.ridx;
-  s.rdata = S.data;
-  s.rdata_it = -1;
-  s.rec = nullptr;
-  s.rec_it = -1;
-  s.snap = nullptr;
-  s.illum = S.illum;
-  s.isnap = nullptr;
-  s.image = nullptr;
-  s.w = c.w;
-  const long long n = (long long)S.nx * S.nz;
-  const float* const* model;
-  if (fwd) {
-    const int it = m;
-    model = S.mf;
-    if (pass == 0) {
-      s.src_it = it;
-      if (it % S.k == 0) s.snap = S.snaps + (long long)(it / S.k) * n;
+  const int ms = fwd ? m : m - S.nt;  // step within this propagation
+  const bool odd = ms & 1;
+  float* const* cur = fwd ? (odd ? S.f2 : S.f) : (odd ? S.fb2 : S.fb);
+  float* const* nxt = fwd ? (odd ? S.f : S.f2) : (odd ? S.fb : S.fb2);
+  const float* const* model = fwd ? S.mf : S.mb;
+  const int it = fwd ? m : 2 * S.nt - 1 - m;
+  const long long n = (long long)nx * nz;
+  // imaging steps: forward vz snapshot + illumination, backward correlation
+  // with the snapshot of the same step (cursor == it/k)
+  gfloat* snap = nullptr;
+  const gfloat* isnap = nullptr;
+  if (it % S.k == 0) {
+    if (fwd) snap = as_global(S.snaps) + (long long)(it / S.k) * n;
+    else isnap = as_global((const float*)S.snaps) + (long long)(it / S.k) * n;
+  }
+  const int zlo = c.fs ? 1 : 2;  // first interior row
+  // Every global read of the step is issued here, before the first barrier,
+  // from clamped addresses (so no load sits behind a branch and waits alone);
+  // cells outside the grid are then set to 0.
+  const int gxc = min(max(gx, 0), nx - 1);
+  {
+    const gfloat* oxx = as_global((const float*)cur[2]);
+    const gfloat* ozz = as_global((const float*)cur[3]);
+    const gfloat* oxz = as_global((const float*)cur[4]);
+    float a[kFSQ], bb[kFSQ], d[kFSQ];
+#pragma unroll
+    for (int q = 0; q < kFSQ; ++q) {
+      const int gz = z0 - 4 + wv + q * kFWaves;
+      const int i = min(max(gz, 0), nz - 1) * nx + gxc;
+      a[q] = oxx[i];
+      bb[q] = ozz[i];
+      d[q] = oxz[i];
     }
-  } else {
-    const int it = 2 * S.nt - 1 - m;
-    model = S.mb;
-    if (pass == 0) {
-      s.rdata_it = it;
-      if (it % S.k == 0) {  // cursor == it/k for (nt-1)/k+1 snapshots
-        s.isnap = S.snaps + (long long)(it / S.k) * n;
-        s.image = S.image;
-      }
+#pragma unroll
+    for (int q = 0; q < kFSQ; ++q) {
+      const int r = wv + q * kFWaves, gz = z0 - 4 + r;
+      const bool in = xin && gz >= 0 && gz < nz;
+      s_xx[r][lane] = in ? a[q] : 0.0f;
+      s_zz[r][lane] = in ? bb[q] : 0.0f;
+      s_xz[r][lane] = in ? d[q] : 0.0f;
     }
   }
-  s.lam = model[0];
-  s.mu = model[1];
-  s.bx = model[2];
-  s.bz = model[3];
-  // every row's reads first, then the rows' updates (see ela_vel_load)
-  if (pass == 0) {
-    ElaVelIn in[kERows];
+  // velocity rows: old velocities, buoyancies, sponge factors and the imaging
+  // operands; core rows: Lame parameters and sponge factors
+  float ovx[kFVQ], ovz[kFVQ], obx[kFVQ], obz[kFVQ], gv[kFVQ], side[kFVQ], isn[kFVQ];
+  float lam[kFCQ], mu[kFCQ], gc[kFCQ];
+  const gfloat* tab = as_global(c.table);
+  {
+    const gfloat* vx = as_global((const float*)cur[0]);
+    const gfloat* vz = as_global((const float*)cur[1]);
+    const gfloat* bx = as_global(model[2]);
+    const gfloat* bz = as_global(model[3]);
+    // imaging steps read the illumination (forward) or the image (backward)
+    const gfloat* sp = as_global((const float*)(snap != nullptr ? S.illum : S.image));
+    const bool imaging = snap != nullptr || isnap != nullptr;
 #pragma unroll
-    for (int r = 0; r < kERows; ++r)
-      if (iz0 + r * kEBZ < S.nz) in[r] = ela_vel_load(s, ix, iz0 + r * kEBZ);
+    for (int q = 0; q < kFVQ; ++q) {
+      const int gzc = min(max(z0 - 2 + wv + q * kFWaves, 0), nz - 1);
+      const int i = gzc * nx + gxc;
+      ovx[q] = vx[i];
+      ovz[q] = vz[i];
+      obx[q] = bx[i];
+      obz[q] = bz[i];
+      const int dep = sponge_depth(gxc, gzc, nx, nz, c.n_sponge, c.fs);
+      const float t = tab[dep];
+      gv[q] = dep > 0 ? t : 1.0f;
+      side[q] = isn[q] = 0.0f;
+      if (imaging) side[q] = sp[i];
+      if (isnap != nullptr) isn[q] = isnap[i];
+    }
+    const gfloat* L = as_global(model[0]);
+    const gfloat* M = as_global(model[1]);
 #pragma unroll
-    for (int r = 0; r < kERows; ++r)
-      if (iz0 + r * kEBZ < S.nz) ela_vel_finish(s, ix, iz0 + r * kEBZ, in[r]);
-  } else {
-    ElaStrIn in[kERows];
+    for (int q = 0; q < kFCQ; ++q) {
+      const int gzc = min(z0 + wv + q * kFWaves, nz - 1);
+      const int i = gzc * nx + gxc;
+      lam[q] = L[i];
+      mu[q] = M[i];
+      const int dep = sponge_depth(gxc, gzc, nx, nz, c.n_sponge, c.fs);
+      const float t = tab[dep];
+      gc[q] = dep > 0 ? t : 1.0f;
+    }
+  }
+  // injection operands: the source sample (forward); on the receiver row, the
+  // lane's receiver range and its first sample (back-propagation)
+  const float sval = fwd ? as_global(S.src)[it] : 0.0f;
+  const int rvr = S.riz - (z0 - 2);  // velocity row of the receiver row
+  const bool rwave = !fwd && rvr >= 0 && rvr < kFVR && rvr % kFWaves == wv;
+  int r0 = 0, r1 = 0;
+  float rs0 = 0.0f;
+  const gfloat* rrow = as_global(S.data) + (long long)it * S.nrec;
+  if (rwave) {
+    r0 = as_global(S.roff)[gxc];
+    r1 = as_global(S.roff)[gxc + 1];
+    if (r0 < r1) rs0 = rrow[as_global(S.ridx)[r0]];
+  }
+  __syncthreads();
+  // velocity pass on the core plus 2 cells each side (wave2d.c:123-155), with
+  // the injection and, on core cells, the imaging side work
+  {
+    gfloat* nvx = as_global(nxt[0]);
+    gfloat* nvz = as_global(nxt[1]);
 #pragma unroll
-    for (int r = 0; r < kERows; ++r)
-      if (iz0 + r * kEBZ < S.nz) in[r] = ela_str_load(s, ix, iz0 + r * kEBZ);
+    for (int q = 0; q < kFVQ; ++q) {
+      const int vr = wv + q * kFWaves, lz = vr + 2, gz = z0 - 2 + vr;
+      float vxn = 0.0f, vzn = 0.0f;
+      if (lane >= 2 && lane < kFC - 2 && xin && gz >= 0 && gz < nz) {
+        const int i = gz * nx + gx;
+        const float dtb = c.dt * obz[q];
+        float vzc = ovz[q];
+        if (fwd) {
+          if (i == S.src_cell) vzc = vzc + dtb * sval;
+        } else if (vr == rvr && r0 < r1) {
+          vzc = vzc + dtb * rs0;
+          for (int r = r0 + 1; r < r1; ++r) vzc = vzc + dtb * rrow[as_global(S.ridx)[r]];
+        }
+        if (lane >= 4 && lane < 4 + kFW && vr >= 2 && vr < 2 + kFH) {
+          if (snap != nullptr) {
+            snap[i] = vzc;
+            as_global(S.illum)[i] = side[q] + vzc * vzc;
+          }
+          if (isnap != nullptr) as_global(S.image)[i] = side[q] + (isn[q] * vzc) * c.w;
+        }
+        if (!(gx < 2 || gx >= nx - 2 || gz < zlo || gz >= nz - 2)) {
+          ElaVelIn v;
+          v.upd = true;
+          v.vz0 = ovz[q];
+          v.vxi = ovx[q];
+          v.bxi = obx[q];
+          v.bzi = obz[q];
+          v.sxx_p1 = s_xx[lz][lane + 1];
+          v.sxx_0 = s_xx[lz][lane];
+          v.sxx_p2 = s_xx[lz][lane + 2];
+          v.sxx_m1 = s_xx[lz][lane - 1];
+          v.sxz_0 = s_xz[lz][lane];
+          v.sxz_mz = s_xz[lz - 1][lane];
+          v.sxz_pz = s_xz[lz + 1][lane];
+          v.sxz_m2z = s_xz[lz - 2][lane];
+          v.sxz_m1 = s_xz[lz][lane - 1];
+          v.sxz_p1 = s_xz[lz][lane + 1];
+          v.sxz_m2 = s_xz[lz][lane - 2];
+          v.szz_pz = s_zz[lz + 1][lane];
+          v.szz_0 = s_zz[lz][lane];
+          v.szz_p2z = s_zz[lz + 2][lane];
+          v.szz_mz = s_zz[lz - 1][lane];
+          v.g = gv[q];
+          ela_vel_update(v, vzc, c.rdx, c.rdz, c.dt, vxn, vzn);
+        } else {
+          vxn = ovx[q];
+          vzn = vzc;
+        }
+        if (lane >= 4 && lane < 4 + kFW && vr >= 2 && vr < 2 + kFH) {
+          nvx[i] = vxn;
+          nvz[i] = vzn;
+        }
+      }
+      s_vx[vr][lane] = vxn;
+      s_vz[vr][lane] = vzn;
+    }
+  }
+  __syncthreads();
+  // stress pass on the core (wave2d.c:158-195) and the free surface on row 0
+  // (wave2d.c:198-211); cells the pass does not update keep their values
+  {
+    gfloat* nxx = as_global(nxt[2]);
+    gfloat* nzz = as_global(nxt[3]);
+    gfloat* nxz = as_global(nxt[4]);
 #pragma unroll
-    for (int r = 0; r < kERows; ++r)
-      if (iz0 + r * kEBZ < S.nz) ela_str_finish(s, ix, iz0 + r * kEBZ, in[r]);
+    for (int q = 0; q < kFCQ; ++q) {
+      const int cr = wv + q * kFWaves, gz = z0 + cr, lz = cr + 4, vr = cr + 2;
+      if (!core_x || gz >= nz) continue;
+      const int i = gz * nx + gx;
+      float sxx = s_xx[lz][lane], szz = s_zz[lz][lane], sxz = s_xz[lz][lane];
+      if (gz == 0) {
+        if (c.fs) {
+          szz = 0.0f;
+          sxz = 0.0f;
+          if (gx >= 2 && gx < nx - 2)
+            sxx = ela_free_surface_sxx(lam[q], mu[q], s_vx[vr][lane - 2], s_vx[vr][lane - 1],
+                                       s_vx[vr][lane], s_vx[vr][lane + 1], c.rdx);
+        }
+      } else if (!(gx < 2 || gx >= nx - 2 || gz < zlo || gz >= nz - 2)) {
+        ElaStrIn v;
+        v.upd = true;
+        v.vx_0 = s_vx[vr][lane];
+        v.vx_m1 = s_vx[vr][lane - 1];
+        v.vx_p1 = s_vx[vr][lane + 1];
+        v.vx_m2 = s_vx[vr][lane - 2];
+        v.vx_pz = s_vx[vr + 1][lane];
+        v.vx_p2z = s_vx[vr + 2][lane];
+        v.vx_mz = s_vx[vr - 1][lane];
+        v.vz_0 = s_vz[vr][lane];
+        v.vz_mz = s_vz[vr - 1][lane];
+        v.vz_pz = s_vz[vr + 1][lane];
+        v.vz_m2z = s_vz[vr - 2][lane];
+        v.vz_p1 = s_vz[vr][lane + 1];
+        v.vz_p2 = s_vz[vr][lane + 2];
+        v.vz_m1 = s_vz[vr][lane - 1];
+        v.l = lam[q];
+        v.m = mu[q];
+        v.sxx = sxx;
+        v.szz = szz;
+        v.sxz = sxz;
+        v.g = gc[q];
+        ela_str_update(v, c.rdx, c.rdz, c.dt, sxx, szz, sxz);
+      }
+      nxx[i] = sxx;
+      nzz[i] = szz;
+      nxz[i] = sxz;
+    }
   }
 }
 
-void ela_batch_launch(const EConst& c, const EShot* shots, const int* list, int nact,
-                      int ntiles, int g, int pass, hipStream_t st) {
-  hipLaunchKernelGGL(k_ela_batch, dim3(ntiles), dim3(kEBX, kEBZ), 0, st, c, shots, list, nact,
-                     ntiles, g, pass);
+void ela_batch_step(const EConst& c, const EShot* shots, const int* list, int nact, int ntiles,
+                    int g, hipStream_t st) {
+  hipLaunchKernelGGL(k_ela_fused, dim3(ntiles), dim3(kFC, kFWaves), 0, st, c, shots, list, nact,
+                     ntiles, g);
 }
 
 int ela_tiles(int nx, int nz, int* tiles_x) {
-  *tiles_x = (nx + kEBX - 1) / kEBX;
-  return *tiles_x * ((nz + kETileZ - 1) / kETileZ);
+  *tiles_x = (nx + kFW - 1) / kFW;
+  return *tiles_x * ((nz + kFH - 1) / kFH);
 }
 
 __global__ void k_ela_scale(float* p, long long n, float w) {
@@ -189,10 +345,7 @@ static int ealloc(ElaLane* L, DevBuf& d, size_t bytes) {
 }
 
 static void e_launch(ElaLane* L, const EEpoch& e, int g, hipStream_t st) {
-  for (int pass = 0; pass < 2; ++pass)
-    hipLaunchKernelGGL(k_ela_batch, dim3(e.tiles), dim3(kEBX, kEBZ), 0, st, L->c,
-                       (const EShot*)L->d_shots.p, (const int*)e.d_list, e.nshot, e.tiles, g,
-                       pass);
+  ela_batch_step(L->c, (const EShot*)L->d_shots.p, (const int*)e.d_list, e.nshot, e.tiles, g, st);
 }
 
 static const EEpoch* e_epoch_at(const ElaLane* L, int g) {
@@ -313,13 +466,12 @@ int ela_lane_create(const std::vector<const tsw_shot_desc*>& shots, double dx, d
     S.src_cell = d.siz * d.nx + d.six;
     S.riz = d.riz;
     S.nrec = d.nrec;
-    S.tiles_x = (d.nx + kEBX - 1) / kEBX;
-    S.tiles = S.tiles_x * ((d.nz + kETileZ - 1) / kETileZ);
+    S.tiles = ela_tiles(d.nx, d.nz, &S.tiles_x);
     L->cell_updates += 2LL * d.nt * d.nx * d.nz;
   }
   if ((rc = e_build(L))) return fail(rc);
-  // per slot: 5 forward + 5 back-propagation fields + vz snapshot arena, sized
-  // for the slot's largest shot
+  // per slot: two sets each of the 5 forward and the 5 back-propagation fields
+  // + vz snapshot arena, sized for the slot's largest shot
   std::vector<long long> cells(L->slots, 0), snap(L->slots, 0);
   for (int i = 0; i < L->nshot; ++i) {
     const long long n = (long long)L->sh[i].nx * L->sh[i].nz;
@@ -327,11 +479,11 @@ int ela_lane_create(const std::vector<const tsw_shot_desc*>& shots, double dx, d
     cells[s] = std::max(cells[s], n);
     snap[s] = std::max(snap[s], (long long)((L->sh[i].nt - 1) / k + 1) * n);
   }
-  L->per_slot.resize((size_t)L->slots * 11);
+  L->per_slot.resize((size_t)L->slots * 21);
   for (int s = 0; s < L->slots; ++s) {
-    for (int j = 0; j < 10; ++j)
-      if ((rc = ealloc(L, L->per_slot[s * 11 + j], cells[s] * sizeof(float)))) return fail(rc);
-    if ((rc = ealloc(L, L->per_slot[s * 11 + 10], snap[s] * sizeof(float)))) return fail(rc);
+    for (int j = 0; j < 20; ++j)
+      if ((rc = ealloc(L, L->per_slot[s * 21 + j], cells[s] * sizeof(float)))) return fail(rc);
+    if ((rc = ealloc(L, L->per_slot[s * 21 + 20], snap[s] * sizeof(float)))) return fail(rc);
   }
   // per shot: 8 model arrays, src, data, roff, ridx, illum, image
   L->per_shot.resize((size_t)L->nshot * 14);
@@ -372,10 +524,12 @@ int ela_lane_create(const std::vector<const tsw_shot_desc*>& shots, double dx, d
     S.image = P[13].f();
     const int s = L->slot_of[i];
     for (int j = 0; j < 5; ++j) {
-      S.f[j] = L->per_slot[s * 11 + j].f();
-      S.fb[j] = L->per_slot[s * 11 + 5 + j].f();
+      S.f[j] = L->per_slot[s * 21 + j].f();
+      S.fb[j] = L->per_slot[s * 21 + 5 + j].f();
+      S.f2[j] = L->per_slot[s * 21 + 10 + j].f();
+      S.fb2[j] = L->per_slot[s * 21 + 15 + j].f();
     }
-    S.snaps = L->per_slot[s * 11 + 10].f();
+    S.snaps = L->per_slot[s * 21 + 20].f();
   }
   if ((rc = ealloc(L, L->d_shots, L->sh.size() * sizeof(EShot)))) return fail(rc);
   if (hipMemcpy(L->d_shots.p, L->sh.data(), L->sh.size() * sizeof(EShot), hipMemcpyHostToDevice))
@@ -455,17 +609,13 @@ int ela_lane_time(ElaLane* L, double* ms, int* launches) {
   for (int g = 0; g < L->total_steps && rc == kOk; ++g) {
     const EEpoch* e = e_epoch_at(L, g);
     if (!e || e->tiles == 0) continue;
-    for (int pass = 0; pass < 2 && rc == kOk; ++pass) {
-      if (inrun == 0 && hipEventRecord(ev[2 * used], st) != hipSuccess) rc = kErrHip;
-      hipLaunchKernelGGL(k_ela_batch, dim3(e->tiles), dim3(kEBX, kEBZ), 0, st, L->c,
-                         (const EShot*)L->d_shots.p, (const int*)e->d_list, e->nshot, e->tiles,
-                         g, pass);
-      ++count;
-      if (++inrun == kTimedRun) {
-        if (hipEventRecord(ev[2 * used + 1], st) != hipSuccess) rc = kErrHip;
-        inrun = 0;
-        if (++used == chunk && rc == kOk) rc = drain();
-      }
+    if (inrun == 0 && hipEventRecord(ev[2 * used], st) != hipSuccess) rc = kErrHip;
+    e_launch(L, *e, g, st);
+    ++count;
+    if (++inrun == kTimedRun) {
+      if (hipEventRecord(ev[2 * used + 1], st) != hipSuccess) rc = kErrHip;
+      inrun = 0;
+      if (++used == chunk && rc == kOk) rc = drain();
     }
   }
   if (rc == kOk && inrun) {

@@ -62,6 +62,21 @@ struct ElaVelIn {
   float szz_pz, szz_0, szz_p2z, szz_mz;
 };
 
+// Interior velocity update from the stencil values (wave2d.c:127-152); vzc is
+// the cell's vz after injection.
+__device__ __forceinline__ void ela_vel_update(const ElaVelIn& v, float vzc, float rdx, float rdz,
+                                               float dt, float& vx, float& vz) {
+  const float a = TSW_S1, b = TSW_S2;
+  const float dsxx = (a * (v.sxx_p1 - v.sxx_0) + b * (v.sxx_p2 - v.sxx_m1)) * rdx;
+  const float dsxz_z = (a * (v.sxz_0 - v.sxz_mz) + b * (v.sxz_pz - v.sxz_m2z)) * rdz;
+  vx = v.vxi + (dt * v.bxi) * (dsxx + dsxz_z);
+  const float dsxz_x = (a * (v.sxz_0 - v.sxz_m1) + b * (v.sxz_p1 - v.sxz_m2)) * rdx;
+  const float dszz = (a * (v.szz_pz - v.szz_0) + b * (v.szz_p2z - v.szz_mz)) * rdz;
+  vz = vzc + (dt * v.bzi) * (dsxz_x + dszz);
+  vx = vx * v.g;
+  vz = vz * v.g;
+}
+
 __device__ __forceinline__ ElaVelIn ela_vel_load(const ElaStep& s, int ix, int iz) {
   const int nx = s.nx;
   const int i = iz * nx + ix;
@@ -132,15 +147,8 @@ __device__ __forceinline__ void ela_vel_finish(const ElaStep& s, int ix, int iz,
     if (injected) as_global(s.vz)[i] = vzc;
     return;
   }
-  const float a = TSW_S1, b = TSW_S2;
-  const float dsxx = (a * (v.sxx_p1 - v.sxx_0) + b * (v.sxx_p2 - v.sxx_m1)) * s.rdx;
-  const float dsxz_z = (a * (v.sxz_0 - v.sxz_mz) + b * (v.sxz_pz - v.sxz_m2z)) * s.rdz;
-  float vx = v.vxi + (s.dt * v.bxi) * (dsxx + dsxz_z);
-  const float dsxz_x = (a * (v.sxz_0 - v.sxz_m1) + b * (v.sxz_p1 - v.sxz_m2)) * s.rdx;
-  const float dszz = (a * (v.szz_pz - v.szz_0) + b * (v.szz_p2z - v.szz_mz)) * s.rdz;
-  float vz = vzc + (s.dt * v.bzi) * (dsxz_x + dszz);
-  vx = vx * v.g;
-  vz = vz * v.g;
+  float vx, vz;
+  ela_vel_update(v, vzc, s.rdx, s.rdz, s.dt, vx, vz);
   as_global(s.vx)[i] = vx;
   as_global(s.vz)[i] = vz;
 }
@@ -167,6 +175,32 @@ struct ElaStrIn {
   float vz_0, vz_mz, vz_pz, vz_m2z, vz_p1, vz_p2, vz_m1;
   float l, m, sxx, szz, sxz, g;
 };
+
+// Interior stress update from the stencil values (wave2d.c:162-192).
+__device__ __forceinline__ void ela_str_update(const ElaStrIn& v, float rdx, float rdz, float dt,
+                                               float& sxx, float& szz, float& sxz) {
+  const float a = TSW_S1, b = TSW_S2;
+  const float dvx_x = (a * (v.vx_0 - v.vx_m1) + b * (v.vx_p1 - v.vx_m2)) * rdx;
+  const float dvz_z = (a * (v.vz_0 - v.vz_mz) + b * (v.vz_pz - v.vz_m2z)) * rdz;
+  const float l2m = v.l + 2.0f * v.m;
+  sxx = v.sxx + dt * (l2m * dvx_x + v.l * dvz_z);
+  szz = v.szz + dt * (v.l * dvx_x + l2m * dvz_z);
+  const float dvz_x = (a * (v.vz_p1 - v.vz_0) + b * (v.vz_p2 - v.vz_m1)) * rdx;
+  const float dvx_z = (a * (v.vx_pz - v.vx_0) + b * (v.vx_p2z - v.vx_mz)) * rdz;
+  sxz = v.sxz + (dt * v.m) * (dvz_x + dvx_z);
+  sxx = sxx * v.g;
+  szz = szz * v.g;
+  sxz = sxz * v.g;
+}
+
+// sxx on the free-surface row from the updated vx of row 0 at ix-2..ix+1
+// (wave2d.c:198-211): an assignment without dt, as the reference does.
+__device__ __forceinline__ float ela_free_surface_sxx(float l, float m, float vx_m2, float vx_m1,
+                                                      float vx_0, float vx_p1, float rdx) {
+  const float a = TSW_S1, b = TSW_S2;
+  const float d = (a * (vx_0 - vx_m1) + b * (vx_p1 - vx_m2)) * rdx;
+  return (((4.0f * m) * (l + m)) / ((l + 2.0f * m) + 1e-8f)) * d;
+}
 
 __device__ __forceinline__ ElaStrIn ela_str_load(const ElaStep& s, int ix, int iz) {
   const int nx = s.nx;
@@ -208,7 +242,6 @@ __device__ __forceinline__ ElaStrIn ela_str_load(const ElaStep& s, int ix, int i
 __device__ __forceinline__ void ela_str_finish(const ElaStep& s, int ix, int iz,
                                                const ElaStrIn& v) {
   const int nx = s.nx;
-  const float a = TSW_S1, b = TSW_S2;
   if (s.rec_it >= 0 && iz == s.riz) {
     const int r0 = as_global(s.roff)[ix], r1 = as_global(s.roff)[ix + 1];
     if (r0 < r1) {
@@ -219,30 +252,19 @@ __device__ __forceinline__ void ela_str_finish(const ElaStep& s, int ix, int iz,
   }
   if (iz == 0) {
     if (!s.fs) return;
-    // assignment without dt, as the reference does
     const gfloat* vx = as_global((const float*)s.vx);
     as_global(s.szz)[ix] = 0.0f;
     as_global(s.sxz)[ix] = 0.0f;
-    if (ix >= 2 && ix < nx - 2) {
-      const float l = as_global(s.lam)[ix], m = as_global(s.mu)[ix];
-      const float d = (a * (vx[ix] - vx[ix - 1]) + b * (vx[ix + 1] - vx[ix - 2])) * s.rdx;
-      as_global(s.sxx)[ix] = (((4.0f * m) * (l + m)) / ((l + 2.0f * m) + 1e-8f)) * d;
-    }
+    if (ix >= 2 && ix < nx - 2)
+      as_global(s.sxx)[ix] = ela_free_surface_sxx(as_global(s.lam)[ix], as_global(s.mu)[ix],
+                                                  vx[ix - 2], vx[ix - 1], vx[ix], vx[ix + 1],
+                                                  s.rdx);
     return;
   }
   if (!v.upd) return;
   const int i = iz * nx + ix;
-  const float dvx_x = (a * (v.vx_0 - v.vx_m1) + b * (v.vx_p1 - v.vx_m2)) * s.rdx;
-  const float dvz_z = (a * (v.vz_0 - v.vz_mz) + b * (v.vz_pz - v.vz_m2z)) * s.rdz;
-  const float l2m = v.l + 2.0f * v.m;
-  float sxx = v.sxx + s.dt * (l2m * dvx_x + v.l * dvz_z);
-  float szz = v.szz + s.dt * (v.l * dvx_x + l2m * dvz_z);
-  const float dvz_x = (a * (v.vz_p1 - v.vz_0) + b * (v.vz_p2 - v.vz_m1)) * s.rdx;
-  const float dvx_z = (a * (v.vx_pz - v.vx_0) + b * (v.vx_p2z - v.vx_mz)) * s.rdz;
-  float sxz = v.sxz + (s.dt * v.m) * (dvz_x + dvx_z);
-  sxx = sxx * v.g;
-  szz = szz * v.g;
-  sxz = sxz * v.g;
+  float sxx, szz, sxz;
+  ela_str_update(v, s.rdx, s.rdz, s.dt, sxx, szz, sxz);
   as_global(s.sxx)[i] = sxx;
   as_global(s.szz)[i] = szz;
   as_global(s.sxz)[i] = sxz;

@@ -107,7 +107,7 @@ static long long req_heavy(const Req& r, int k) {
   long long b = up(4LL * r.nt) + up(4LL * r.nt * r.nrec) + up(4LL * (r.nx + 1)) +
                 up(4LL * std::max(r.nrec, 1)) + up(4 * nsnap * n);
   if (r.kind == TSW_SHOT_ACOUSTIC) b += 2 * nb + nb /* staging */ + 6 * nb;
-  else b += 8 * up(4 * n) + 10 * nb;
+  else b += 8 * up(4 * n) + 20 * nb;  // models, two sets of 5 + 5 fields
   return b;
 }
 static long long req_keep(const Req& r) { return 2 * (long long)Arena::up((size_t)(4 * r.cells())); }
@@ -212,6 +212,8 @@ static int admit(Server* S, Req* r, int s) {
     for (int j = 0; j < 5; ++j) {
       E.f[j] = (float*)take(nb);
       E.fb[j] = (float*)take(nb);
+      E.f2[j] = (float*)take(nb);
+      E.fb2[j] = (float*)take(nb);
     }
     E.snaps = (float*)take((size_t)nsnap * n * sizeof(float));
     E.nx = r->nx;
@@ -340,9 +342,8 @@ static int tick(Server* S, std::vector<Req*>& done, long long& cu, double& busy)
   const double t_start = S->now();
   for (size_t q = 0; q < el.size(); ++q) {
     if (el[q].second == 0) continue;
-    for (int pass = 0; pass < 2; ++pass)
-      ela_batch_launch(S->ec, (const EShot*)S->d_eshots.p, S->d_elist.i() + el[q].first,
-                       el[q].second, etiles[q], S->g_e + (int)q, pass, S->se);
+    ela_batch_step(S->ec, (const EShot*)S->d_eshots.p, S->d_elist.i() + el[q].first,
+                   el[q].second, etiles[q], S->g_e + (int)q, S->se);
   }
   for (size_t L = 0; L < lr.size(); ++L)
     tb_launch(S->kernel, S->bc, (const TbShot*)S->d_tbshots.p,
