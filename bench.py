@@ -385,6 +385,29 @@ def preprocessing_timing(log):
     out["total_ms"] = round(total * 1e3, 2)
     out["first_block_ms"] = round(first * 1e3, 2)
     out["input_channel_samples_per_s"] = round(nt * nch / total, 1)
+    # the same chain kept on the device (one copy of the block in), then the
+    # block's correlations (64 sources x 800 receivers, lags -3.5..3.5 s)
+    # added into a resident stack, as stacked_correlations runs them
+    import torch
+
+    from das_amd.interferometry import cross_correlations_device
+    from das_amd.preprocessing import preprocess_block_device
+
+    srcs = np.linspace(20, nch - 20, 64).astype(np.int32)
+    w, dtp, _ = preprocess_block_device(x, 100.0, 50.0, 0.8, 4.5)
+    stack, _ = cross_correlations_device(w, srcs, 3.5, dtp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    w, dtp, _ = preprocess_block_device(x, 100.0, 50.0, 0.8, 4.5, out=w)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    cross_correlations_device(w, srcs, 3.5, dtp, out=stack)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    out["device_chain_ms"] = round((t1 - t0) * 1e3, 2)
+    out["device_chain_plus_correlation_ms"] = round((t2 - t0) * 1e3, 2)
+    log(f"preprocessing on the device: {(t1 - t0) * 1e3:.1f} ms per block, "
+        f"{(t2 - t0) * 1e3:.1f} ms with its correlations into the stack")
     return out
 
 

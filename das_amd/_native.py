@@ -79,6 +79,7 @@ SIGNATURES = {
     "tsw_sosfiltfilt": (I, [P, I, I, P, I, P, I, I, I, P]),
     "tsw_sign": (I, [P, LL, P]),
     "tsw_spectral_whitening": (I, [P, I, I, D, D, D, D, P]),
+    "tsw_preprocess_device": (I, [P, I, I, P, I, P, I, I, P, I, P, I, D, D, D, D, P, P]),
     "tsw_fk_filter": (I, [P, I, I, D, D, D, D, P]),
     "tsw_save_matrix_csv": (I, [ctypes.c_char_p, P, I, I, I, P, P, ctypes.c_char_p,
                                 ctypes.c_char_p]),
@@ -130,6 +131,15 @@ def lib():
                 if not os.path.exists(LIB_PATH):
                     raise FileNotFoundError(
                         f"HIP engine not built: {LIB_PATH} (run `python -m das_amd.build`)")
+                # torch first: its HIP runtime is then the one the engine's
+                # hip* symbols bind to, so the engine and the tensors whose
+                # device pointers and streams it takes share one runtime (an
+                # engine loaded first binds the system runtime, and torch's own
+                # then finds no device)
+                try:
+                    import torch  # noqa: F401
+                except ImportError:
+                    pass
                 L = ctypes.CDLL(LIB_PATH)
                 for name, (res, args) in SIGNATURES.items():
                     fn = getattr(L, name)

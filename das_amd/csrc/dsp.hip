@@ -24,7 +24,7 @@
 namespace tsw {
 
 constexpr int kMaxSections = 16;
-constexpr int kSosChunk = 16;  // samples read ahead of the recursion
+constexpr int kSosChunk = 32;  // samples per chunk read ahead of the recursion
 
 struct SosArgs {
   const float* x;   // (nt, nch) row-major
@@ -35,25 +35,38 @@ struct SosArgs {
   float* out;       // (ceil(nt / step), nch)
 };
 
-// extended sample e of channel c (odd extension, float32 as numpy forms it)
-__device__ __forceinline__ float sos_ext(const SosArgs& a, int c, int e) {
+// Extended sample e of channel c (odd extension, float32 as numpy forms it),
+// in two halves so that a chunk's loads can all be issued before any value is
+// used: sos_ext_at is the raw sample the value is formed from (any e, clamped
+// into the block, no branch around the load) and sos_ext_val forms the value.
+__device__ __forceinline__ float sos_ext_at(const SosArgs& a, int c, int e) {
   const int n = a.nt, p = a.padlen;
-  const float* x = a.x;
-  const long long s = a.nch;
-  if (e < p) return 2.0f * x[c] - x[(long long)(p - e) * s + c];
-  if (e < p + n) return x[(long long)(e - p) * s + c];
-  const int j = e - p - n;  // 0..p-1
-  return 2.0f * x[(long long)(n - 1) * s + c] - x[(long long)(n - 2 - j) * s + c];
+  int r = e < p ? p - e : (e < p + n ? e - p : n - 2 - (e - p - n));
+  r = min(max(r, 0), n - 1);  // samples past the end of the last chunk
+  return a.x[(long long)r * a.nch + c];
+}
+
+__device__ __forceinline__ float sos_ext_val(const SosArgs& a, int e, float raw, float first,
+                                             float last) {
+  if (e < a.padlen) return 2.0f * first - raw;
+  if (e < a.padlen + a.nt) return raw;
+  return 2.0f * last - raw;
 }
 
 // One thread per channel: consecutive channels are consecutive addresses.
 // NS > 0: exactly NS sections, unrolled, with the state and coefficients in
 // registers; NS == 0: any count up to kMaxSections (state in private memory).
+// The recursion is sequential in time; its input is read in chunks of
+// kSosChunk samples, two chunks in flight (A and B): a chunk's loads are
+// issued before the previous chunk's recursion, so the memory latency is
+// hidden behind it.  A 64-thread block is one wave and there are few of them
+// (one per 64 channels), so the kernel may use the whole register file.
 template <int NS>
-__global__ void k_sosfiltfilt(SosArgs a) {
+__global__ __launch_bounds__(64, 1) void k_sosfiltfilt(SosArgs a) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.nch) return;
   constexpr int MS = NS > 0 ? NS : kMaxSections;
+  constexpr int K = kSosChunk;
   const int nsec = NS > 0 ? NS : a.nsec;
   const int ne = a.nt + 2 * a.padlen;
   double z[MS][2], b0[MS], b1[MS], b2[MS], a1[MS], a2[MS];
@@ -65,7 +78,8 @@ __global__ void k_sosfiltfilt(SosArgs a) {
     a1[s] = a.sos[s][4];
     a2[s] = a.sos[s][5];
   }
-  const double x0 = (double)sos_ext(a, c, 0);
+  const float first = a.x[c], last = a.x[(long long)(a.nt - 1) * a.nch + c];
+  const double x0 = (double)sos_ext_val(a, 0, sos_ext_at(a, c, 0), first, last);
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
     z[s][0] = a.zi[s][0] * x0;
@@ -83,19 +97,29 @@ __global__ void k_sosfiltfilt(SosArgs a) {
     }
     return xn;
   };
-  // The recursion is sequential in time; the samples of each chunk of kSosChunk
-  // are read before the chunk's recursion so that one memory latency covers
-  // the chunk instead of one per sample.
+  // forward pass over the extended samples, into tmp
   double y = 0.0;
-  for (int e0 = 0; e0 < ne; e0 += kSosChunk) {
-    float xs[kSosChunk];
+  {
+    float xa[K], xb[K];
+    auto load = [&](float* buf, int e0) {
 #pragma unroll
-    for (int u = 0; u < kSosChunk; ++u) xs[u] = e0 + u < ne ? sos_ext(a, c, e0 + u) : 0.0f;
+      for (int u = 0; u < K; ++u) buf[u] = sos_ext_at(a, c, e0 + u);
+    };
+    auto run = [&](const float* buf, int e0) {
 #pragma unroll
-    for (int u = 0; u < kSosChunk; ++u) {
-      if (e0 + u >= ne) break;
-      y = cascade((double)xs[u]);
-      a.tmp[(long long)(e0 + u) * a.nch + c] = y;
+      for (int u = 0; u < K; ++u) {
+        const int e = e0 + u;
+        if (e >= ne) break;
+        y = cascade((double)sos_ext_val(a, e, buf[u], first, last));
+        a.tmp[(long long)e * a.nch + c] = y;
+      }
+    };
+    load(xa, 0);
+    for (int e0 = 0; e0 < ne; e0 += 2 * K) {
+      load(xb, e0 + K);
+      run(xa, e0);
+      load(xa, e0 + 2 * K);
+      run(xb, e0 + K);
     }
   }
 #pragma unroll
@@ -103,22 +127,33 @@ __global__ void k_sosfiltfilt(SosArgs a) {
     z[s][0] = a.zi[s][0] * y;
     z[s][1] = a.zi[s][1] * y;
   }
-  for (int e1 = ne - 1; e1 >= 0; e1 -= kSosChunk) {
-    double ts[kSosChunk];
+  // reversed pass over tmp (written by this thread only), every step-th sample out
+  {
+    double ta[K], tb[K];
+    auto load = [&](double* buf, int e1) {
 #pragma unroll
-    for (int u = 0; u < kSosChunk; ++u)
-      ts[u] = e1 - u >= 0 ? a.tmp[(long long)(e1 - u) * a.nch + c] : 0.0;
+      for (int u = 0; u < K; ++u) buf[u] = a.tmp[(long long)max(e1 - u, 0) * a.nch + c];
+    };
+    auto run = [&](const double* buf, int e1) {
 #pragma unroll
-    for (int u = 0; u < kSosChunk; ++u) {
-      const int e = e1 - u;
-      if (e < 0) break;
-      const double xn = cascade(ts[u]);
-      const int t = e - a.padlen;
-      if (t >= 0 && t < a.nt && t % a.step == 0) {
-        float v = (float)xn;
-        if (a.sign) v = v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f);
-        a.out[(long long)(t / a.step) * a.nch + c] = v;
+      for (int u = 0; u < K; ++u) {
+        const int e = e1 - u;
+        if (e < 0) break;
+        const double xn = cascade(buf[u]);
+        const int t = e - a.padlen;
+        if (t >= 0 && t < a.nt && t % a.step == 0) {
+          float v = (float)xn;
+          if (a.sign) v = v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f);
+          a.out[(long long)(t / a.step) * a.nch + c] = v;
+        }
       }
+    };
+    load(ta, ne - 1);
+    for (int e1 = ne - 1; e1 >= 0; e1 -= 2 * K) {
+      load(tb, e1 - K);
+      run(ta, e1);
+      load(ta, e1 - 2 * K);
+      run(tb, e1 - K);
     }
   }
 }
@@ -218,7 +253,7 @@ static unsigned gridn(long long n) {
 // largest call.  Every entry point holds the library mutex, which guards them.
 struct DspCache {
   std::map<std::array<int, 3>, hipfftHandle> plans;  // (kind, n0, n1) -> plan
-  DevBuf buf[3];
+  DevBuf buf[4];
 };
 
 static DspCache& dsp_cache() {
@@ -256,27 +291,27 @@ static int cached_plan(int kind, int n0, int n1, hipfftHandle* out) {
 
 static DevBuf& scratch(int i) { return dsp_cache().buf[i]; }
 
-}  // namespace tsw
-
-using namespace tsw;
-
-extern "C" {
-
-int tsw_sosfiltfilt(const float* x, int nt, int nch, const double* sos, int nsec,
-                    const double* zi, int padlen, int step, int sign, float* out) {
-  std::lock_guard<std::recursive_mutex> lk(api_mutex());
-  TSW_ARG(x && sos && zi && out, "null argument");
+static int sos_check(int nt, int nch, const double* sos, int nsec, int padlen, int step) {
+  TSW_ARG(sos != nullptr, "null argument");
   TSW_ARG(nt >= 1 && nch >= 1 && step >= 1, "bad sizes");
   TSW_ARG(nsec >= 1 && nsec <= kMaxSections, "1 to %d second-order sections", kMaxSections);
   TSW_ARG(padlen >= 0 && padlen < nt,
           "the input must be longer than padlen (%d samples, as sosfiltfilt requires)", padlen);
-  Ctx* ctx;
-  TSW_TRY(ctx_get(&ctx));
-  hipStream_t st = ctx->stream;
+  for (int s = 0; s < nsec; ++s)
+    TSW_ARG(sos[s * 6 + 3] == 1.0, "section %d is not normalised (a0 = %g)", s, sos[s * 6 + 3]);
+  return kOk;
+}
+
+// Zero-phase filtering of the (nt, nch) device array d_x into d_out
+// (ceil(nt / step) rows), optionally the sign of each output sample; the
+// padded FP64 work array is scratch buffer 1.
+static int sos_run(const float* d_x, int nt, int nch, const double* sos, int nsec,
+                   const double* zi, int padlen, int step, int sign, float* d_out,
+                   hipStream_t st) {
+  TSW_ARG(zi != nullptr, "null argument");
   SosArgs a;
   std::memset(&a, 0, sizeof a);
   for (int s = 0; s < nsec; ++s) {
-    TSW_ARG(sos[s * 6 + 3] == 1.0, "section %d is not normalised (a0 = %g)", s, sos[s * 6 + 3]);
     for (int j = 0; j < 6; ++j) a.sos[s][j] = sos[s * 6 + j];
     a.zi[s][0] = zi[s * 2];
     a.zi[s][1] = zi[s * 2 + 1];
@@ -287,15 +322,11 @@ int tsw_sosfiltfilt(const float* x, int nt, int nch, const double* sos, int nsec
   a.padlen = padlen;
   a.step = step;
   a.sign = sign ? 1 : 0;
-  const int nout = (nt + step - 1) / step;
-  DevBuf &dx = scratch(0), &dtmp = scratch(1), &dout = scratch(2);
-  TSW_TRY(dx.ensure((size_t)nt * nch * sizeof(float)));
+  DevBuf& dtmp = scratch(1);
   TSW_TRY(dtmp.ensure((size_t)(nt + 2 * padlen) * nch * sizeof(double)));
-  TSW_TRY(dout.ensure((size_t)nout * nch * sizeof(float)));
-  TSW_HIP(hipMemcpyAsync(dx.p, x, (size_t)nt * nch * sizeof(float), hipMemcpyHostToDevice, st));
-  a.x = dx.f();
+  a.x = d_x;
   a.tmp = (double*)dtmp.p;
-  a.out = dout.f();
+  a.out = d_out;
   const dim3 grid((nch + 63) / 64), block(64);
   switch (nsec) {  // the usual cascades (order 2 to 16) unrolled
     case 1: hipLaunchKernelGGL(k_sosfiltfilt<1>, grid, block, 0, st, a); break;
@@ -308,6 +339,53 @@ int tsw_sosfiltfilt(const float* x, int nt, int nch, const double* sos, int nsec
     default: hipLaunchKernelGGL(k_sosfiltfilt<0>, grid, block, 0, st, a); break;
   }
   TSW_HIP(hipGetLastError());
+  return kOk;
+}
+
+// Spectral whitening of the (nt, nch) device array d_x into d_out (may be
+// d_x): X(f) / (|X(f)| + eps) for fmin <= f <= fmax, 0 outside; the spectrum
+// is scratch buffer 3.
+static int whiten_run(float* d_x, int nt, int nch, double fs, double fmin, double fmax,
+                      double eps, float* d_out, hipStream_t st) {
+  const int nf = nt / 2 + 1;
+  // bins k with fmin <= k fs / nt <= fmax
+  const int k0 = std::max(0, (int)std::ceil(fmin * nt / fs - 1e-9));
+  const int k1 = std::min(nf - 1, (int)std::floor(fmax * nt / fs + 1e-9));
+  DevBuf& dX = scratch(3);
+  TSW_TRY(dX.ensure((size_t)nf * nch * sizeof(hipfftComplex)));
+  hipfftHandle fwd, inv;
+  TSW_TRY(cached_plan(0, nt, nch, &fwd));
+  TSW_TRY(cached_plan(1, nt, nch, &inv));
+  TSW_FFT(hipfftSetStream(fwd, st));
+  TSW_FFT(hipfftSetStream(inv, st));
+  TSW_FFT(hipfftExecR2C(fwd, d_x, (hipfftComplex*)dX.p));
+  hipLaunchKernelGGL(k_whiten, dim3(gridn((long long)nf * nch)), dim3(256), 0, st,
+                     (hipfftComplex*)dX.p, nf, nch, k0, k1, (float)eps, (float)nt);
+  TSW_HIP(hipGetLastError());
+  TSW_FFT(hipfftExecC2R(inv, (hipfftComplex*)dX.p, d_out));
+  return kOk;
+}
+
+}  // namespace tsw
+
+using namespace tsw;
+
+extern "C" {
+
+int tsw_sosfiltfilt(const float* x, int nt, int nch, const double* sos, int nsec,
+                    const double* zi, int padlen, int step, int sign, float* out) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  TSW_ARG(x && out, "null argument");
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  hipStream_t st = ctx->stream;
+  const int nout = (nt + step - 1) / step;
+  DevBuf &dx = scratch(0), &dout = scratch(2);
+  TSW_TRY(sos_check(nt, nch, sos, nsec, padlen, step));
+  TSW_TRY(dx.ensure((size_t)nt * nch * sizeof(float)));
+  TSW_TRY(dout.ensure((size_t)nout * nch * sizeof(float)));
+  TSW_HIP(hipMemcpyAsync(dx.p, x, (size_t)nt * nch * sizeof(float), hipMemcpyHostToDevice, st));
+  TSW_TRY(sos_run(dx.f(), nt, nch, sos, nsec, zi, padlen, step, sign, dout.f(), st));
   TSW_HIP(hipMemcpyAsync(out, dout.p, (size_t)nout * nch * sizeof(float), hipMemcpyDeviceToHost,
                          st));
   TSW_HIP(hipStreamSynchronize(st));
@@ -340,25 +418,48 @@ int tsw_spectral_whitening(const float* x, int nt, int nch, double fs, double fm
   Ctx* ctx;
   TSW_TRY(ctx_get(&ctx));
   hipStream_t st = ctx->stream;
-  const int nf = nt / 2 + 1;
-  // bins k with fmin <= k fs / nt <= fmax
-  const int k0 = std::max(0, (int)std::ceil(fmin * nt / fs - 1e-9));
-  const int k1 = std::min(nf - 1, (int)std::floor(fmax * nt / fs + 1e-9));
-  DevBuf &dx = scratch(0), &dX = scratch(1);
+  DevBuf& dx = scratch(0);
   TSW_TRY(dx.ensure((size_t)nt * nch * sizeof(float)));
-  TSW_TRY(dX.ensure((size_t)nf * nch * sizeof(hipfftComplex)));
   TSW_HIP(hipMemcpyAsync(dx.p, x, (size_t)nt * nch * sizeof(float), hipMemcpyHostToDevice, st));
-  hipfftHandle fwd, inv;
-  TSW_TRY(cached_plan(0, nt, nch, &fwd));
-  TSW_TRY(cached_plan(1, nt, nch, &inv));
-  TSW_FFT(hipfftSetStream(fwd, st));
-  TSW_FFT(hipfftSetStream(inv, st));
-  TSW_FFT(hipfftExecR2C(fwd, dx.f(), (hipfftComplex*)dX.p));
-  hipLaunchKernelGGL(k_whiten, dim3(gridn((long long)nf * nch)), dim3(256), 0, st,
-                     (hipfftComplex*)dX.p, nf, nch, k0, k1, (float)eps, (float)nt);
-  TSW_HIP(hipGetLastError());
-  TSW_FFT(hipfftExecC2R(inv, (hipfftComplex*)dX.p, dx.f()));
+  TSW_TRY(whiten_run(dx.f(), nt, nch, fs, fmin, fmax, eps, dx.f(), st));
   TSW_HIP(hipMemcpyAsync(out, dx.p, (size_t)nt * nch * sizeof(float), hipMemcpyDeviceToHost, st));
+  TSW_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+int tsw_preprocess_device(const float* x, int nt, int nch, const double* lp_sos, int lp_nsec,
+                          const double* lp_zi, int lp_padlen, int q, const double* bp_sos,
+                          int bp_nsec, const double* bp_zi, int bp_padlen, double fs_proc,
+                          double fmin, double fmax, double eps, float* d_out, void* stream) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  TSW_ARG(x && d_out && bp_sos && bp_zi, "null argument");
+  TSW_ARG(nt >= 1 && nch >= 1 && q >= 1, "bad sizes");
+  TSW_ARG(fs_proc > 0 && fmin <= fmax && eps >= 0, "bad band");
+  const int nt1 = q > 1 ? (nt + q - 1) / q : nt;  // samples after decimation
+  if (q > 1) {
+    TSW_ARG(lp_sos && lp_zi, "null argument");
+    TSW_TRY(sos_check(nt, nch, lp_sos, lp_nsec, lp_padlen, q));
+  }
+  TSW_TRY(sos_check(nt1, nch, bp_sos, bp_nsec, bp_padlen, 1));
+  TSW_ARG(nt1 >= 2, "the decimated block needs at least 2 samples");
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DevBuf &a = scratch(0), &b = scratch(2);
+  TSW_TRY(a.ensure((size_t)nt * nch * sizeof(float)));
+  TSW_TRY(b.ensure((size_t)nt1 * nch * sizeof(float)));
+  TSW_HIP(hipMemcpyAsync(a.p, x, (size_t)nt * nch * sizeof(float), hipMemcpyHostToDevice, st));
+  float* cur = a.f();
+  float* other = b.f();
+  if (q > 1) {  // low-pass at the new Nyquist frequency, every q-th sample
+    TSW_TRY(sos_run(a.f(), nt, nch, lp_sos, lp_nsec, lp_zi, lp_padlen, q, 0, b.f(), st));
+    cur = b.f();
+    other = a.f();
+  }
+  // band-pass with the 1-bit normalisation of its output, then whitening
+  TSW_TRY(sos_run(cur, nt1, nch, bp_sos, bp_nsec, bp_zi, bp_padlen, 1, 1, other, st));
+  TSW_TRY(whiten_run(other, nt1, nch, fs_proc, fmin, fmax, eps, d_out, st));
+  // the scratch buffers are free again once the stream has drained
   TSW_HIP(hipStreamSynchronize(st));
   return kOk;
 }

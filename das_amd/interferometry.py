@@ -48,6 +48,20 @@ def _n_lags(max_lag_seconds: float, dt: float) -> int:
     return int(round(max_lag_seconds / dt))
 
 
+def _xcorr_args(nch, source_channels, receiver_indices, max_lag_seconds, dt):
+    """Channel lists (negative indices from the end, checked), lag count and
+    output shape of the correlations."""
+    rec = np.arange(nch) if receiver_indices is None else np.asarray(receiver_indices)
+    rec = np.ascontiguousarray(np.where(rec < 0, rec + nch, rec), dtype=np.int32)
+    src = np.asarray(source_channels).reshape(-1)
+    src = np.ascontiguousarray(np.where(src < 0, src + nch, src), dtype=np.int32)
+    for a, what in ((src, "source_channel_idx"), (rec, "receiver_indices")):
+        if a.size and (a.min() < 0 or a.max() >= nch):
+            raise IndexError(f"{what} out of range for {nch} channels")
+    nlag = _n_lags(max_lag_seconds, dt)
+    return src, rec, nlag, (len(src), 2 * nlag + 1, len(rec))
+
+
 def cross_correlations(data: np.ndarray, source_channels: Sequence[int], max_lag_seconds: float,
                        dt: float, receiver_indices: Optional[np.ndarray] = None,
                        out: Optional[np.ndarray] = None) -> Tuple[np.ndarray, np.ndarray]:
@@ -59,15 +73,8 @@ def cross_correlations(data: np.ndarray, source_channels: Sequence[int], max_lag
     if d.ndim != 2:
         raise ValueError("data must be (nt, n_channels)")
     nt, nch = d.shape
-    rec = np.arange(nch) if receiver_indices is None else np.asarray(receiver_indices)
-    rec = np.ascontiguousarray(np.where(rec < 0, rec + nch, rec), dtype=np.int32)
-    src = np.asarray(source_channels).reshape(-1)
-    src = np.ascontiguousarray(np.where(src < 0, src + nch, src), dtype=np.int32)
-    for a, what in ((src, "source_channel_idx"), (rec, "receiver_indices")):
-        if a.size and (a.min() < 0 or a.max() >= nch):
-            raise IndexError(f"{what} out of range for {nch} channels")
-    nlag = _n_lags(max_lag_seconds, dt)
-    shape = (len(src), 2 * nlag + 1, len(rec))
+    src, rec, nlag, shape = _xcorr_args(nch, source_channels, receiver_indices, max_lag_seconds,
+                                        dt)
     acc = 0
     if out is None:
         out = np.empty(shape, np.float32)
@@ -79,6 +86,34 @@ def cross_correlations(data: np.ndarray, source_channels: Sequence[int], max_lag
                               acc, N.ptr(out)))
     lags = np.arange(-nlag, nlag + 1) * dt
     return out, lags
+
+
+def cross_correlations_device(data, source_channels: Sequence[int], max_lag_seconds: float,
+                              dt: float, receiver_indices: Optional[np.ndarray] = None,
+                              out=None):
+    """cross_correlations of a (nt, nch) float32 CUDA tensor into a CUDA
+    tensor; with ``out`` the correlations are added into it in FP32 (stacking
+    over files on the device) -> (out, lags)."""
+    import torch
+
+    if data.ndim != 2 or data.dtype != torch.float32 or not data.is_cuda \
+            or not data.is_contiguous():
+        raise ValueError("data must be a contiguous (nt, n_channels) float32 CUDA tensor")
+    nt, nch = data.shape
+    src, rec, nlag, shape = _xcorr_args(nch, source_channels, receiver_indices, max_lag_seconds,
+                                        dt)
+    acc = 0
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=data.device)
+    else:
+        if tuple(out.shape) != shape or out.dtype != torch.float32 or not out.is_cuda \
+                or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous float32 CUDA tensor of shape {shape}")
+        acc = 1
+    N.check(N.lib().tsw_xcorr_device(data.data_ptr(), nt, nch, N.ptr(src), len(src), N.ptr(rec),
+                                     len(rec), nlag, acc, out.data_ptr(),
+                                     torch.cuda.current_stream(data.device).cuda_stream))
+    return out, np.arange(-nlag, nlag + 1) * dt
 
 
 def compute_cross_correlation(data: np.ndarray, source_channel_idx: int, max_lag_seconds: float,

@@ -6,7 +6,10 @@ virtual-source channel with the receiver channels, stacked over blocks and
 divided by the block count; per source the causal Green's function (time
 derivative) and an F-K fan filter; resampling to the simulation step; and
 multi-shot migration with Ricker sources.  Each step is the das_amd function
-of the same name as the reference's.
+of the same name as the reference's; per block, the preprocessing and the
+correlation run back to back on the device (preprocess_block_device,
+cross_correlations_device), so a block crosses PCIe once and the stack stays
+in HBM until the last block.
 """
 from __future__ import annotations
 
@@ -14,10 +17,9 @@ from typing import Iterable, List, Sequence, Tuple
 
 import numpy as np
 
-from .interferometry import build_virtual_shot_gather, cross_correlations, \
+from .interferometry import build_virtual_shot_gather, cross_correlations_device, \
     resample_gather_to_sim_dt
-from .preprocessing import bandpass_filter, decimate_time, fk_filter, spectral_whitening, \
-    temporal_normalization
+from .preprocessing import fk_filter, preprocess_block_device
 from .rtm_engine import RTMResult, TSWRTMEngine
 from .velocity_flood import FloodModels
 from .wave_solver import ricker_wavelet
@@ -29,20 +31,17 @@ def stacked_correlations(blocks: Iterable[np.ndarray], fs_orig: float, target_fs
                          ) -> Tuple[np.ndarray, np.ndarray, float]:
     """run_das_pipeline.py:160-213 and :232-235 -> (ccfs (n_src, 2 n_lags + 1,
     n_rec) float32 averaged over the blocks, lags, dt_proc)."""
-    stacked, lags, dt_proc, n = None, None, None, 0
+    stacked, lags, dt_proc, n, white = None, None, None, 0, None
     for raw in blocks:
-        dec, dt_proc, fs_proc = decimate_time(raw, fs_orig, target_fs)
-        bp = bandpass_filter(dec, fs_proc, fmin, fmax)
-        white = spectral_whitening(temporal_normalization(bp, "1bit"), fs_proc, fmin, fmax)
-        if stacked is None:
-            stacked, lags = cross_correlations(white, source_channels, max_lag_s, dt_proc,
-                                               receiver_channels)
-        else:
-            cross_correlations(white, source_channels, max_lag_s, dt_proc, receiver_channels,
-                               out=stacked)
+        # decimation, band-pass, 1-bit and whitening on the device; the block
+        # crosses PCIe once and the stack stays resident until the end
+        white, dt_proc, _ = preprocess_block_device(raw, fs_orig, target_fs, fmin, fmax, out=white)
+        stacked, lags = cross_correlations_device(white, source_channels, max_lag_s, dt_proc,
+                                                  receiver_channels, out=stacked)
         n += 1
     if stacked is None:
         raise ValueError("no data blocks")
+    stacked = stacked.cpu().numpy()
     return stacked / np.float32(n) if n > 1 else stacked, lags, dt_proc
 
 

@@ -17,11 +17,8 @@ from scipy import signal
 from . import _native as N
 
 
-def _sosfiltfilt(x, sos, step=1, sign=False):
-    x = N.f32(x)
-    if x.ndim == 1:
-        return _sosfiltfilt(x[:, None], sos, step, sign)[:, 0]
-    nt, nch = x.shape
+def _sos_args(sos, nt):
+    """(sos, zi, nsec, padlen) for the zero-phase filter of nt samples."""
     sos = np.ascontiguousarray(sos, dtype=np.float64)
     zi = np.ascontiguousarray(signal.sosfilt_zi(sos), dtype=np.float64)
     nsec = sos.shape[0]
@@ -30,6 +27,15 @@ def _sosfiltfilt(x, sos, step=1, sign=False):
     if nt <= padlen:
         raise ValueError(f"The length of the input vector x must be greater than padlen, "
                          f"which is {padlen}.")
+    return sos, zi, nsec, padlen
+
+
+def _sosfiltfilt(x, sos, step=1, sign=False):
+    x = N.f32(x)
+    if x.ndim == 1:
+        return _sosfiltfilt(x[:, None], sos, step, sign)[:, 0]
+    nt, nch = x.shape
+    sos, zi, nsec, padlen = _sos_args(sos, nt)
     out = np.empty(((nt + step - 1) // step, nch), np.float32)
     N.check(N.lib().tsw_sosfiltfilt(N.ptr(x), nt, nch, N.ptr(sos), nsec, N.ptr(zi), padlen,
                                     int(step), 1 if sign else 0, N.ptr(out)))
@@ -94,3 +100,39 @@ def fk_filter(gather: np.ndarray, dt: float, dx: float, vmin: float, vmax: float
     N.check(N.lib().tsw_fk_filter(N.ptr(g), nt, nx, float(dt), float(dx), float(vmin),
                                   float(vmax), N.ptr(out)))
     return out
+
+
+def preprocess_block_device(data: np.ndarray, fs_orig: float, target_fs: float, fmin: float,
+                            fmax: float, out=None, eps: float = 1e-10):
+    """decimate_time, bandpass_filter, temporal_normalization("1bit") and
+    spectral_whitening of one (nt, nch) block (run_das_pipeline.py:165-168)
+    in one call that copies the block to the device once and leaves the result
+    there: -> (torch float32 CUDA tensor (nt_proc, nch), dt_proc, fs_proc),
+    bit-identical to the four functions in turn.  ``out`` (a CUDA float32
+    tensor of that shape) is reused when given."""
+    import torch
+
+    x = N.f32(data)
+    if x.ndim != 2:
+        raise ValueError("data must be (nt, n_channels)")
+    nt, nch = x.shape
+    q = int(round(fs_orig / target_fs))
+    lp = (None, None, 0, 0)
+    if q > 1:
+        fs_proc = fs_orig / q
+        lp = _sos_args(signal.butter(8, fs_proc / 2.0, btype="lowpass", fs=fs_orig,
+                                     output="sos"), nt)
+    else:
+        q, fs_proc = 1, float(fs_orig)
+    nt1 = (nt + q - 1) // q
+    bp = _sos_args(signal.butter(4, [fmin, fmax], btype="bandpass", fs=fs_proc, output="sos"),
+                   nt1)
+    if out is None or tuple(out.shape) != (nt1, nch) or out.dtype != torch.float32 \
+            or not out.is_cuda or not out.is_contiguous():
+        out = torch.empty((nt1, nch), dtype=torch.float32, device="cuda")
+    ptr = lambda a: None if a is None else N.ptr(a)  # noqa: E731
+    N.check(N.lib().tsw_preprocess_device(
+        N.ptr(x), nt, nch, ptr(lp[0]), lp[2], ptr(lp[1]), lp[3], q, N.ptr(bp[0]), bp[2],
+        N.ptr(bp[1]), bp[3], float(fs_proc), float(fmin), float(fmax), float(eps), out.data_ptr(),
+        torch.cuda.current_stream(out.device).cuda_stream))
+    return out, 1.0 / fs_proc, fs_proc

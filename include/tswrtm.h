@@ -349,6 +349,19 @@ int tsw_sign(const float* x, long long n, float* out);
  * fmin <= f <= fmax and 0 elsewhere, irfft (FP32 transforms). */
 int tsw_spectral_whitening(const float* x, int nt, int nch, double fs, double fmin, double fmax,
                            double eps, float* out);
+/* The chain of one block on the device (run_das_pipeline.py:160-200, the
+ * steps of README.md:27-52): x (nt, nch) on the host is copied once, low-pass
+ * filtered and decimated by q (lp_*: as tsw_sosfiltfilt; skipped when q <= 1,
+ * then lp_* may be null), band-pass filtered with the sign of each output
+ * sample (bp_*), whitened at fs_proc between fmin and fmax, and left in d_out
+ * (ceil(nt/q), nch) float32 on the device, for tsw_xcorr_device.  The results
+ * equal tsw_sosfiltfilt (step q), tsw_sosfiltfilt (sign), then
+ * tsw_spectral_whitening, bit for bit.  Runs on `stream` and returns once it
+ * has drained. */
+int tsw_preprocess_device(const float* x, int nt, int nch, const double* lp_sos, int lp_nsec,
+                          const double* lp_zi, int lp_padlen, int q, const double* bp_sos,
+                          int bp_nsec, const double* bp_zi, int bp_padlen, double fs_proc,
+                          double fmin, double fmax, double eps, float* d_out, void* stream);
 /* F-K fan filter of a gather g (nt, nx): 2-D FFT, keep wavenumbers k != 0 with
  * vmin <= |f| / |k| <= vmax, inverse FFT, real part (FP32 transforms). */
 int tsw_fk_filter(const float* g, int nt, int nx, double dt, double dx, double vmin, double vmax,

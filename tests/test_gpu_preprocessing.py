@@ -108,3 +108,61 @@ def test_transform_plans_per_shape():
         np.testing.assert_array_equal(spectral_whitening(a, 50.0, 1.0, 10.0), wa)
         np.testing.assert_array_equal(fk_filter(b, 0.02, 8.0, 100.0, 900.0), fb)
         np.testing.assert_array_equal(fk_filter(a, 0.02, 8.0, 100.0, 900.0), fa)
+
+
+@pytest.mark.parametrize("fs_orig,target,nt,nch", [(100.0, 50.0, 3001, 37), (200.0, 50.0, 6000, 16),
+                                                   (50.0, 50.0, 2000, 9)])
+def test_device_chain_equals_step_by_step(fs_orig, target, nt, nch):
+    """One call that keeps the block on the device (decimation, band-pass
+    with the 1-bit sign fused into its output, whitening) equals the four
+    host-pointer functions in turn, bit for bit; with and without decimation,
+    odd block lengths, a reused output tensor."""
+    from das_amd.preprocessing import (bandpass_filter, decimate_time, preprocess_block_device,
+                                       spectral_whitening, temporal_normalization)
+
+    out = None
+    for seed in (1, 2):
+        x = _noise(nt, nch, seed=seed, scale=5.0)
+        dec, dt, fs = decimate_time(x, fs_orig, target)
+        want = spectral_whitening(temporal_normalization(bandpass_filter(dec, fs, 0.8, 4.5)), fs,
+                                  0.8, 4.5)
+        got, dt2, fs2 = preprocess_block_device(x, fs_orig, target, 0.8, 4.5, out=out)
+        assert (dt2, fs2) == (dt, fs)
+        assert out is None or got.data_ptr() == out.data_ptr()
+        out = got
+        np.testing.assert_array_equal(got.cpu().numpy(), want)
+
+
+def test_device_chain_rejects_short_blocks_like_scipy():
+    from das_amd.preprocessing import preprocess_block_device
+
+    with pytest.raises(ValueError, match="padlen"):
+        preprocess_block_device(_noise(40, 2), 100.0, 50.0, 0.8, 4.5)
+
+
+def test_stacked_correlations_equal_host_chain():
+    """The virtual-source stack built on the device (each block copied once,
+    the stack resident until the end) equals the host-pointer chain with the
+    FP32 stack accumulated on the host, bit for bit."""
+    from das_amd.interferometry import cross_correlations
+    from das_amd.pipeline import stacked_correlations
+    from das_amd.preprocessing import (bandpass_filter, decimate_time, spectral_whitening,
+                                       temporal_normalization)
+
+    nch = 40
+    blocks = [_noise(4000, nch, seed=s) for s in (4, 5, 6)]
+    srcs, recs = [3, -2], np.arange(0, nch, 3)
+    ccfs, lags, dtp = stacked_correlations(blocks, 100.0, 50.0, 0.8, 4.5, srcs, recs, 2.0)
+    stack = None
+    for raw in blocks:
+        dec, dt, fs = decimate_time(raw, 100.0, 50.0)
+        w = spectral_whitening(temporal_normalization(bandpass_filter(dec, fs, 0.8, 4.5)), fs,
+                               0.8, 4.5)
+        if stack is None:
+            stack, want_lags = cross_correlations(w, srcs, 2.0, dt, recs)
+        else:
+            cross_correlations(w, srcs, 2.0, dt, recs, out=stack)
+    assert dtp == dt
+    np.testing.assert_array_equal(lags, want_lags)
+    np.testing.assert_array_equal(ccfs, stack / np.float32(len(blocks)))
+    assert np.abs(ccfs).max() > 0
