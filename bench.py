@@ -92,14 +92,24 @@ def elastic_leg(args, log):
     b.close()
     cu = cfg.cell_updates
     ach = ELASTIC_BYTES_PER_CELL_UPDATE * cu / (ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_ela_fused (velocity and stress passes, one launch per step)",
+            "launches": nl, "avg_launch_us": round(ms * 1e3 / max(nl, 1), 3),
+            "algorithmic_bytes_per_cell_update": ELASTIC_BYTES_PER_CELL_UPDATE}
+    tp = os.path.join(ROOT, "profiles", "elastic_step_hbm_traffic.json")
+    if os.path.exists(tp):
+        # HBM bytes the step kernel moves per cell-update (rocprofv3 FETCH_SIZE +
+        # WRITE_SIZE on a shortened run of this shot), at this run's kernel time
+        with open(tp) as f:
+            per = json.load(f)["step_total_B_per_cell_update"]
+        gbs = per * cu / (ms * 1e-3) / 1e9
+        roof["traffic"] = {"bytes_per_cell_update": per, "gbs": round(gbs, 1),
+                           "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                           "source": "profiles/elastic_step_hbm_traffic.json"}
     return {"workload": "elastic P-SV RTM, 1 shot, 400x1600, nt=8000, dt=0.4 ms, 800 receivers",
             "value": round(cu / el / 1e9, 3), "unit": "Gcell-updates/s",
-            "ms_per_shot": round(el * 1e3, 2),
-            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                         "kernel": "k_ela_fused (velocity and stress passes, one launch per step)", "launches": nl,
-                         "avg_launch_us": round(ms * 1e3 / max(nl, 1), 3),
-                         "algorithmic_bytes_per_cell_update": ELASTIC_BYTES_PER_CELL_UPDATE}}
+            "ms_per_shot": round(el * 1e3, 2), "roofline": roof}
 
 
 def served_leg(args, acoustic_gcell, elastic_gcell, kernel, log):
