@@ -37,8 +37,9 @@ __global__ void k_take_cols(const float* data, int nt, int nch, const int* ch, i
   }
 }
 
-// samples per LDS chunk: 256 keeps a wave's LDS at about 5 KB, so LDS does not
-// cap the waves per CU (1024: 14 KB per wave, 11 waves per CU, 11% slower)
+// samples per LDS chunk: 256 keeps a wave's LDS at about 7 KB, so LDS does not
+// cap the waves per CU (1024 samples staged as FP32 took 14 KB per wave, 11
+// waves per CU, and ran 11% slower)
 constexpr int kXC = 256;
 
 // pairs[(src * nrec + rec) * nl + j] = sum_t S[src][t] * R[rec][t - nlag + j]
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(64) void k_xcorr(const float* S, const float* R, in
   constexpr int kW = 64 * L;
   constexpr int kC = (kXC / L) * L;  // chunk: a whole number of L-sample steps
   __shared__ double s_c[kC];  // source samples converted once (read as broadcasts)
-  __shared__ float r_c[kC + kW + 2 * L];
+  __shared__ double r_c[kC + kW + 2 * L];  // receiver samples converted once
   const int rec = blockIdx.x, src = blockIdx.y;
   const int nl = 2 * nlag + 1;
   const int j0 = blockIdx.z * kW;
@@ -72,15 +73,15 @@ __global__ __launch_bounds__(64) void k_xcorr(const float* S, const float* R, in
     }
     for (int i = threadIdx.x; i < kC + kW + 2 * L; i += 64) {
       const long long t = (long long)t0 + lag_lo + i;
-      r_c[i] = (t >= 0 && t < nt) ? r[t] : 0.0f;
+      r_c[i] = (t >= 0 && t < nt) ? (double)r[t] : 0.0;
     }
     __syncthreads();
     double w[2 * L];
 #pragma unroll
-    for (int q = 0; q < L; ++q) w[q] = (double)r_c[base + q];
+    for (int q = 0; q < L; ++q) w[q] = r_c[base + q];
     for (int i = 0; i < kC; i += L) {
 #pragma unroll
-      for (int q = 0; q < L; ++q) w[L + q] = (double)r_c[i + base + L + q];
+      for (int q = 0; q < L; ++q) w[L + q] = r_c[i + base + L + q];
 #pragma unroll
       for (int u = 0; u < L; ++u) {
         const double sv = s_c[i + u];
