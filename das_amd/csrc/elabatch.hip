@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kFC* kFWaves) void k_ela_fused(EConst c, const ESho
       gv[q] = dep > 0 ? t : 1.0f;
       side[q] = isn[q] = 0.0f;
       if (imaging) side[q] = sp[i];
-      if (isnap != nullptr) isn[q] = isnap[i];
+      if (isnap != nullptr) isn[q] = __builtin_nontemporal_load(isnap + i);  // read once
     }
     const gfloat* L = as_global(model[0]);
     const gfloat* M = as_global(model[1]);
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(kFC* kFWaves) void k_ela_fused(EConst c, const ESho
         }
         if (lane >= 4 && lane < 4 + kFW && vr >= 2 && vr < 2 + kFH) {
           if (snap != nullptr) {
-            snap[i] = vzc;
+            __builtin_nontemporal_store(vzc, snap + i);  // read back steps later
             as_global(S.illum)[i] = side[q] + vzc * vzc;
           }
           if (isnap != nullptr) as_global(S.image)[i] = side[q] + (isn[q] * vzc) * c.w;
