@@ -42,24 +42,36 @@ __global__ void k_take_cols(const float* data, int nt, int nch, const int* ch, i
 // waves per CU, and ran 11% slower)
 constexpr int kXC = 256;
 
+// rows of S (nrow, nt) float32 into S64 (nrow, ntp) float64, zero past nt
+__global__ void k_rows_f64(const float* S, int nt, int ntp, int nrow, double* S64) {
+  const long long n = (long long)nrow * ntp;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int row = (int)(e / ntp), t = (int)(e - (long long)row * ntp);
+    S64[e] = t < nt ? (double)S[(long long)row * nt + t] : 0.0;
+  }
+}
+
 // pairs[(src * nrec + rec) * nl + j] = sum_t S[src][t] * R[rec][t - nlag + j]
 // (nl = 2 nlag + 1; samples outside [0, nt) are zero).  Grid (nrec, nsrc,
 // lag tiles of 64 L), one wavefront per workgroup, L lags per lane.  The
-// product of two FP32 samples is exact in FP64, so fma(s, r, acc) is the
-// product added with one rounding: the sum is the same as a separate
-// multiply and add, in one instruction.
+// source samples are the same for every lane: they come from S64 (FP64, rows
+// padded with zeros to ntp >= nt + chunk) through scalar loads, so
+// only the receiver window is read from LDS.  The product of two FP32 samples
+// is exact in FP64, so fma(s, r, acc) is the product added with one rounding:
+// the sum is the same as a separate multiply and add, in one instruction.
 template <int L>
-__global__ __launch_bounds__(64) void k_xcorr(const float* S, const float* R, int nt, int nlag,
+__global__ __launch_bounds__(64) void k_xcorr(const double* __restrict__ S64, int ntp,
+                                              const float* __restrict__ R, int nt, int nlag,
                                               int nrec, float* pairs) {
   constexpr int kW = 64 * L;
   constexpr int kC = (kXC / L) * L;  // chunk: a whole number of L-sample steps
-  __shared__ double s_c[kC];  // source samples converted once (read as broadcasts)
   __shared__ double r_c[kC + kW + 2 * L];  // receiver samples converted once
   const int rec = blockIdx.x, src = blockIdx.y;
   const int nl = 2 * nlag + 1;
   const int j0 = blockIdx.z * kW;
   const int lag_lo = -nlag + j0;
-  const float* s = S + (long long)src * nt;
+  const double* s = S64 + (long long)src * ntp;
   const float* r = R + (long long)rec * nt;
   const int base = threadIdx.x * L;
   double acc[L];
@@ -67,15 +79,12 @@ __global__ __launch_bounds__(64) void k_xcorr(const float* S, const float* R, in
   for (int q = 0; q < L; ++q) acc[q] = 0.0;
   for (int t0 = 0; t0 < nt; t0 += kC) {
     __syncthreads();
-    for (int i = threadIdx.x; i < kC; i += 64) {
-      const int t = t0 + i;
-      s_c[i] = t < nt ? (double)s[t] : 0.0;
-    }
     for (int i = threadIdx.x; i < kC + kW + 2 * L; i += 64) {
       const long long t = (long long)t0 + lag_lo + i;
       r_c[i] = (t >= 0 && t < nt) ? (double)r[t] : 0.0;
     }
     __syncthreads();
+    const double* sc = s + t0;
     double w[2 * L];
 #pragma unroll
     for (int q = 0; q < L; ++q) w[q] = r_c[base + q];
@@ -84,7 +93,7 @@ __global__ __launch_bounds__(64) void k_xcorr(const float* S, const float* R, in
       for (int q = 0; q < L; ++q) w[L + q] = r_c[i + base + L + q];
 #pragma unroll
       for (int u = 0; u < L; ++u) {
-        const double sv = s_c[i + u];
+        const double sv = sc[i + u];
 #pragma unroll
         for (int q = 0; q < L; ++q) acc[q] = fma(sv, w[u + q], acc[q]);
       }
@@ -241,14 +250,22 @@ static int xcorr_device(const float* d_data, int nt, int nch, const int* src_ch,
   if (nl <= 256) L = 4;
   else if (nl <= 384) L = 6;
   const unsigned tiles = (unsigned)((nl + 64 * L - 1) / (64 * L));
+  // the source rows in FP64, padded with zeros so that the last chunk (it
+  // starts below nt and spans at most kXC samples) stays inside its row
+  const int ntp = nt + kXC;
+  DevBuf S64;
+  TSW_TRY(S64.ensure((size_t)nsrc * ntp * sizeof(double)));
+  hipLaunchKernelGGL(k_rows_f64, dim3(grid1((long long)nsrc * ntp)), dim3(256), 0, st, S.f(), nt,
+                     ntp, nsrc, (double*)S64.p);
+  const double* s64 = (const double*)S64.p;
   if (L == 4)
-    hipLaunchKernelGGL(k_xcorr<4>, dim3(nrec, nsrc, tiles), dim3(64), 0, st, S.f(), R.f(), nt,
+    hipLaunchKernelGGL(k_xcorr<4>, dim3(nrec, nsrc, tiles), dim3(64), 0, st, s64, ntp, R.f(), nt,
                        nlag, nrec, pairs.f());
   else if (L == 6)
-    hipLaunchKernelGGL(k_xcorr<6>, dim3(nrec, nsrc, tiles), dim3(64), 0, st, S.f(), R.f(), nt,
+    hipLaunchKernelGGL(k_xcorr<6>, dim3(nrec, nsrc, tiles), dim3(64), 0, st, s64, ntp, R.f(), nt,
                        nlag, nrec, pairs.f());
   else
-    hipLaunchKernelGGL(k_xcorr<8>, dim3(nrec, nsrc, tiles), dim3(64), 0, st, S.f(), R.f(), nt,
+    hipLaunchKernelGGL(k_xcorr<8>, dim3(nrec, nsrc, tiles), dim3(64), 0, st, s64, ntp, R.f(), nt,
                        nlag, nrec, pairs.f());
   hipLaunchKernelGGL(k_lag_major, dim3((nl + 31) / 32, (nrec + 31) / 32, nsrc), dim3(32, 8), 0,
                      st, pairs.f(), nrec, nl, d_out, accumulate);
