@@ -37,10 +37,11 @@ __global__ void k_take_cols(const float* data, int nt, int nch, const int* ch, i
   }
 }
 
-// samples per LDS chunk: 256 keeps a wave's LDS at about 7 KB, so LDS does not
-// cap the waves per CU (1024 samples staged as FP32 took 14 KB per wave, 11
-// waves per CU, and ran 11% slower)
-constexpr int kXC = 256;
+// samples per LDS chunk: with L = 6, 96 samples take 3.9 KB of LDS per wave
+// (the chunk plus the 384-lag window, in FP64), so LDS does not cap the waves
+// per CU; measured per 64 x 800-pair launch: 256 samples 22.1 ms, 128 21.4,
+// 96 20.9, 64 21.8 (the smaller the chunk, the more staging and barriers)
+constexpr int kXC = 96;
 
 // rows of S (nrow, nt) float32 into S64 (nrow, ntp) float64, zero past nt
 __global__ void k_rows_f64(const float* S, int nt, int ntp, int nrow, double* S64) {
