@@ -187,8 +187,11 @@ int tsw_migrate_shot_elastic(tsw_grid_t grid, const float* const* model_fwd,
  * (0: no graphs).  Shots are packed into `slots` concurrent lanes (<= 0: all
  * shots at once).  kernel 0: one launch per time step over the tiles of every
  * active shot; kernel c in [1, tsw_kernel_configs()]: temporally blocked
- * launches of configuration c (T steps per launch).  Per-shot results equal
- * tsw_migrate_shot's bit for bit. */
+ * launches of configuration c (T steps per launch).  The elastic lane runs
+ * one launch per time step over every active elastic shot, its velocity and
+ * stress passes fused (each shot holds two sets of its fields, read and
+ * written alternately).  Per-shot results equal tsw_migrate_shot's (elastic:
+ * tsw_migrate_shot_elastic's) bit for bit. */
 int tsw_batch_create(int nshot, const tsw_shot_desc* shots, double dx, double dz, double dt,
                      int n_sponge, double sponge_alpha, int free_surface_top, int k, int slots,
                      int graph_steps, int kernel, tsw_batch_t* out);
