@@ -80,6 +80,7 @@ SIGNATURES = {
     "tsw_sign": (I, [P, LL, P]),
     "tsw_spectral_whitening": (I, [P, I, I, D, D, D, D, P]),
     "tsw_preprocess_device": (I, [P, I, I, P, I, P, I, I, P, I, P, I, D, D, D, D, P, P]),
+    "tsw_preprocess_device_blocks": (I, [I, P, I, I, P, I, P, I, I, P, I, P, I, D, D, D, D, P, P]),
     "tsw_fk_filter": (I, [P, I, I, D, D, D, D, P]),
     "tsw_save_matrix_csv": (I, [ctypes.c_char_p, P, I, I, I, P, P, ctypes.c_char_p,
                                 ctypes.c_char_p]),

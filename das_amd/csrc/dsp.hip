@@ -27,23 +27,24 @@ constexpr int kMaxSections = 16;
 constexpr int kSosChunk = 32;  // samples per chunk read ahead of the recursion
 
 struct SosArgs {
-  const float* x;   // (nt, nch) row-major
-  int nt, nch, nsec, padlen, step, sign;
+  const float* x;   // nblk blocks (nt, nch) row-major, xs elements apart
+  int nt, nch, nsec, padlen, step, sign, nblk;
+  long long xs, os;  // elements between consecutive blocks of x and of out
   double sos[kMaxSections][6];
   double zi[kMaxSections][2];
-  double* tmp;      // (nt + 2 padlen, nch) forward pass
-  float* out;       // (ceil(nt / step), nch)
+  double* tmp;      // (nt + 2 padlen, nblk * nch) forward pass
+  float* out;       // nblk blocks (ceil(nt / step), nch), os elements apart
 };
 
 // Extended sample e of channel c (odd extension, float32 as numpy forms it),
 // in two halves so that a chunk's loads can all be issued before any value is
 // used: sos_ext_at is the raw sample the value is formed from (any e, clamped
 // into the block, no branch around the load) and sos_ext_val forms the value.
-__device__ __forceinline__ float sos_ext_at(const SosArgs& a, int c, int e) {
+__device__ __forceinline__ float sos_ext_at(const SosArgs& a, const float* x, int c, int e) {
   const int n = a.nt, p = a.padlen;
   int r = e < p ? p - e : (e < p + n ? e - p : n - 2 - (e - p - n));
   r = min(max(r, 0), n - 1);  // samples past the end of the last chunk
-  return a.x[(long long)r * a.nch + c];
+  return x[(long long)r * a.nch + c];
 }
 
 __device__ __forceinline__ float sos_ext_val(const SosArgs& a, int e, float raw, float first,
@@ -54,6 +55,8 @@ __device__ __forceinline__ float sos_ext_val(const SosArgs& a, int e, float raw,
 }
 
 // One thread per channel: consecutive channels are consecutive addresses.
+// The channels of nblk blocks (independent recordings of one shape) are
+// filtered side by side: thread g takes channel g % nch of block g / nch.
 // NS > 0: exactly NS sections, unrolled, with the state and coefficients in
 // registers; NS == 0: any count up to kMaxSections (state in private memory).
 // The recursion is sequential in time; its input is read in chunks of
@@ -63,8 +66,12 @@ __device__ __forceinline__ float sos_ext_val(const SosArgs& a, int e, float raw,
 // (one per 64 channels), so the kernel may use the whole register file.
 template <int NS>
 __global__ __launch_bounds__(64, 1) void k_sosfiltfilt(SosArgs a) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.nch) return;
+  const int cg = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cg >= a.nch * a.nblk) return;
+  const int blk = cg / a.nch, c = cg - blk * a.nch;
+  const float* x = a.x + blk * a.xs;
+  float* out = a.out + blk * a.os;
+  const long long tstride = (long long)a.nch * a.nblk;  // tmp row: every block's channels
   constexpr int MS = NS > 0 ? NS : kMaxSections;
   constexpr int K = kSosChunk;
   const int nsec = NS > 0 ? NS : a.nsec;
@@ -78,8 +85,8 @@ __global__ __launch_bounds__(64, 1) void k_sosfiltfilt(SosArgs a) {
     a1[s] = a.sos[s][4];
     a2[s] = a.sos[s][5];
   }
-  const float first = a.x[c], last = a.x[(long long)(a.nt - 1) * a.nch + c];
-  const double x0 = (double)sos_ext_val(a, 0, sos_ext_at(a, c, 0), first, last);
+  const float first = x[c], last = x[(long long)(a.nt - 1) * a.nch + c];
+  const double x0 = (double)sos_ext_val(a, 0, sos_ext_at(a, x, c, 0), first, last);
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
     z[s][0] = a.zi[s][0] * x0;
@@ -103,7 +110,7 @@ __global__ __launch_bounds__(64, 1) void k_sosfiltfilt(SosArgs a) {
     float xa[K], xb[K];
     auto load = [&](float* buf, int e0) {
 #pragma unroll
-      for (int u = 0; u < K; ++u) buf[u] = sos_ext_at(a, c, e0 + u);
+      for (int u = 0; u < K; ++u) buf[u] = sos_ext_at(a, x, c, e0 + u);
     };
     auto run = [&](const float* buf, int e0) {
 #pragma unroll
@@ -111,7 +118,7 @@ __global__ __launch_bounds__(64, 1) void k_sosfiltfilt(SosArgs a) {
         const int e = e0 + u;
         if (e >= ne) break;
         y = cascade((double)sos_ext_val(a, e, buf[u], first, last));
-        a.tmp[(long long)e * a.nch + c] = y;
+        a.tmp[(long long)e * tstride + cg] = y;
       }
     };
     load(xa, 0);
@@ -132,7 +139,7 @@ __global__ __launch_bounds__(64, 1) void k_sosfiltfilt(SosArgs a) {
     double ta[K], tb[K];
     auto load = [&](double* buf, int e1) {
 #pragma unroll
-      for (int u = 0; u < K; ++u) buf[u] = a.tmp[(long long)max(e1 - u, 0) * a.nch + c];
+      for (int u = 0; u < K; ++u) buf[u] = a.tmp[(long long)max(e1 - u, 0) * tstride + cg];
     };
     auto run = [&](const double* buf, int e1) {
 #pragma unroll
@@ -144,7 +151,7 @@ __global__ __launch_bounds__(64, 1) void k_sosfiltfilt(SosArgs a) {
         if (t >= 0 && t < a.nt && t % a.step == 0) {
           float v = (float)xn;
           if (a.sign) v = v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f);
-          a.out[(long long)(t / a.step) * a.nch + c] = v;
+          out[(long long)(t / a.step) * a.nch + c] = v;
         }
       }
     };
@@ -303,11 +310,12 @@ static int sos_check(int nt, int nch, const double* sos, int nsec, int padlen, i
 }
 
 // Zero-phase filtering of the (nt, nch) device array d_x into d_out
-// (ceil(nt / step) rows), optionally the sign of each output sample; the
-// padded FP64 work array is scratch buffer 1.
+// (ceil(nt / step) rows), optionally the sign of each output sample; with
+// nblk > 1, nblk such arrays xs (input) and os (output) elements apart in one
+// launch.  The padded FP64 work array is scratch buffer 1.
 static int sos_run(const float* d_x, int nt, int nch, const double* sos, int nsec,
                    const double* zi, int padlen, int step, int sign, float* d_out,
-                   hipStream_t st) {
+                   hipStream_t st, int nblk = 1, long long xs = 0, long long os = 0) {
   TSW_ARG(zi != nullptr, "null argument");
   SosArgs a;
   std::memset(&a, 0, sizeof a);
@@ -322,12 +330,15 @@ static int sos_run(const float* d_x, int nt, int nch, const double* sos, int nse
   a.padlen = padlen;
   a.step = step;
   a.sign = sign ? 1 : 0;
+  a.nblk = nblk;
+  a.xs = xs;
+  a.os = os;
   DevBuf& dtmp = scratch(1);
-  TSW_TRY(dtmp.ensure((size_t)(nt + 2 * padlen) * nch * sizeof(double)));
+  TSW_TRY(dtmp.ensure((size_t)(nt + 2 * padlen) * nch * nblk * sizeof(double)));
   a.x = d_x;
   a.tmp = (double*)dtmp.p;
   a.out = d_out;
-  const dim3 grid((nch + 63) / 64), block(64);
+  const dim3 grid((nch * nblk + 63) / 64), block(64);
   switch (nsec) {  // the usual cascades (order 2 to 16) unrolled
     case 1: hipLaunchKernelGGL(k_sosfiltfilt<1>, grid, block, 0, st, a); break;
     case 2: hipLaunchKernelGGL(k_sosfiltfilt<2>, grid, block, 0, st, a); break;
@@ -427,13 +438,16 @@ int tsw_spectral_whitening(const float* x, int nt, int nch, double fs, double fm
   return kOk;
 }
 
-int tsw_preprocess_device(const float* x, int nt, int nch, const double* lp_sos, int lp_nsec,
-                          const double* lp_zi, int lp_padlen, int q, const double* bp_sos,
-                          int bp_nsec, const double* bp_zi, int bp_padlen, double fs_proc,
-                          double fmin, double fmax, double eps, float* d_out, void* stream) {
+int tsw_preprocess_device_blocks(int nblk, const float* const* x, int nt, int nch,
+                                 const double* lp_sos, int lp_nsec, const double* lp_zi,
+                                 int lp_padlen, int q, const double* bp_sos, int bp_nsec,
+                                 const double* bp_zi, int bp_padlen, double fs_proc, double fmin,
+                                 double fmax, double eps, float* const* d_out, void* stream) {
   std::lock_guard<std::recursive_mutex> lk(api_mutex());
-  TSW_ARG(x && d_out && bp_sos && bp_zi, "null argument");
+  TSW_ARG(nblk >= 1 && x && d_out && bp_sos && bp_zi, "null argument or no block");
+  for (int b = 0; b < nblk; ++b) TSW_ARG(x[b] && d_out[b], "block %d: null pointer", b);
   TSW_ARG(nt >= 1 && nch >= 1 && q >= 1, "bad sizes");
+  TSW_ARG((long long)nch * nblk < (1LL << 31), "too many channels in one call");
   TSW_ARG(fs_proc > 0 && fmin <= fmax && eps >= 0, "bad band");
   const int nt1 = q > 1 ? (nt + q - 1) / q : nt;  // samples after decimation
   if (q > 1) {
@@ -445,23 +459,40 @@ int tsw_preprocess_device(const float* x, int nt, int nch, const double* lp_sos,
   Ctx* ctx;
   TSW_TRY(ctx_get(&ctx));
   hipStream_t st = static_cast<hipStream_t>(stream);
+  const long long xs = (long long)nt * nch, os = (long long)nt1 * nch;
   DevBuf &a = scratch(0), &b = scratch(2);
-  TSW_TRY(a.ensure((size_t)nt * nch * sizeof(float)));
-  TSW_TRY(b.ensure((size_t)nt1 * nch * sizeof(float)));
-  TSW_HIP(hipMemcpyAsync(a.p, x, (size_t)nt * nch * sizeof(float), hipMemcpyHostToDevice, st));
+  TSW_TRY(a.ensure((size_t)(xs * nblk) * sizeof(float)));
+  TSW_TRY(b.ensure((size_t)(os * nblk) * sizeof(float)));
+  for (int k = 0; k < nblk; ++k)
+    TSW_HIP(hipMemcpyAsync(a.f() + k * xs, x[k], (size_t)xs * sizeof(float),
+                           hipMemcpyHostToDevice, st));
   float* cur = a.f();
   float* other = b.f();
+  long long cs = xs;
   if (q > 1) {  // low-pass at the new Nyquist frequency, every q-th sample
-    TSW_TRY(sos_run(a.f(), nt, nch, lp_sos, lp_nsec, lp_zi, lp_padlen, q, 0, b.f(), st));
+    TSW_TRY(sos_run(a.f(), nt, nch, lp_sos, lp_nsec, lp_zi, lp_padlen, q, 0, b.f(), st, nblk,
+                    xs, os));
     cur = b.f();
     other = a.f();
+    cs = os;
   }
   // band-pass with the 1-bit normalisation of its output, then whitening
-  TSW_TRY(sos_run(cur, nt1, nch, bp_sos, bp_nsec, bp_zi, bp_padlen, 1, 1, other, st));
-  TSW_TRY(whiten_run(other, nt1, nch, fs_proc, fmin, fmax, eps, d_out, st));
+  TSW_TRY(sos_run(cur, nt1, nch, bp_sos, bp_nsec, bp_zi, bp_padlen, 1, 1, other, st, nblk, cs,
+                  os));
+  for (int k = 0; k < nblk; ++k)
+    TSW_TRY(whiten_run(other + k * os, nt1, nch, fs_proc, fmin, fmax, eps, d_out[k], st));
   // the scratch buffers are free again once the stream has drained
   TSW_HIP(hipStreamSynchronize(st));
   return kOk;
+}
+
+int tsw_preprocess_device(const float* x, int nt, int nch, const double* lp_sos, int lp_nsec,
+                          const double* lp_zi, int lp_padlen, int q, const double* bp_sos,
+                          int bp_nsec, const double* bp_zi, int bp_padlen, double fs_proc,
+                          double fmin, double fmax, double eps, float* d_out, void* stream) {
+  return tsw_preprocess_device_blocks(1, &x, nt, nch, lp_sos, lp_nsec, lp_zi, lp_padlen, q,
+                                      bp_sos, bp_nsec, bp_zi, bp_padlen, fs_proc, fmin, fmax,
+                                      eps, &d_out, stream);
 }
 
 int tsw_fk_filter(const float* g, int nt, int nx, double dt, double dx, double vmin, double vmax,

@@ -19,7 +19,7 @@ import numpy as np
 
 from .interferometry import build_virtual_shot_gather, cross_correlations_device, \
     resample_gather_to_sim_dt
-from .preprocessing import fk_filter, preprocess_block_device
+from .preprocessing import fk_filter, preprocess_blocks_device
 from .rtm_engine import RTMResult, TSWRTMEngine
 from .velocity_flood import FloodModels
 from .wave_solver import ricker_wavelet
@@ -27,18 +27,38 @@ from .wave_solver import ricker_wavelet
 
 def stacked_correlations(blocks: Iterable[np.ndarray], fs_orig: float, target_fs: float,
                          fmin: float, fmax: float, source_channels: Sequence[int],
-                         receiver_channels: np.ndarray, max_lag_s: float
-                         ) -> Tuple[np.ndarray, np.ndarray, float]:
+                         receiver_channels: np.ndarray, max_lag_s: float,
+                         group: int = 4) -> Tuple[np.ndarray, np.ndarray, float]:
     """run_das_pipeline.py:160-213 and :232-235 -> (ccfs (n_src, 2 n_lags + 1,
-    n_rec) float32 averaged over the blocks, lags, dt_proc)."""
-    stacked, lags, dt_proc, n, white = None, None, None, 0, None
-    for raw in blocks:
-        # decimation, band-pass, 1-bit and whitening on the device; the block
+    n_rec) float32 averaged over the blocks, lags, dt_proc).  Up to ``group``
+    consecutive blocks of one shape are preprocessed in one call (their
+    channels filtered side by side); the correlations are added into the
+    stack block by block, in order."""
+    stacked, lags, dt_proc, n, whites = None, None, None, 0, None
+
+    def flush(pending):
+        nonlocal stacked, lags, dt_proc, n, whites
+        # decimation, band-pass, 1-bit and whitening on the device; each block
         # crosses PCIe once and the stack stays resident until the end
-        white, dt_proc, _ = preprocess_block_device(raw, fs_orig, target_fs, fmin, fmax, out=white)
-        stacked, lags = cross_correlations_device(white, source_channels, max_lag_s, dt_proc,
-                                                  receiver_channels, out=stacked)
-        n += 1
+        whites, dt_proc, _ = preprocess_blocks_device(pending, fs_orig, target_fs, fmin, fmax,
+                                                      outs=whites)
+        for w in whites[:len(pending)]:
+            stacked, lags = cross_correlations_device(w, source_channels, max_lag_s, dt_proc,
+                                                      receiver_channels, out=stacked)
+            n += 1
+
+    pending = []
+    for raw in blocks:
+        raw = np.asarray(raw)
+        if pending and raw.shape != pending[0].shape:
+            flush(pending)
+            pending = []
+        pending.append(raw)
+        if len(pending) == max(1, int(group)):
+            flush(pending)
+            pending = []
+    if pending:
+        flush(pending)
     if stacked is None:
         raise ValueError("no data blocks")
     stacked = stacked.cpu().numpy()

@@ -102,20 +102,26 @@ def fk_filter(gather: np.ndarray, dt: float, dx: float, vmin: float, vmax: float
     return out
 
 
-def preprocess_block_device(data: np.ndarray, fs_orig: float, target_fs: float, fmin: float,
-                            fmax: float, out=None, eps: float = 1e-10):
+def preprocess_blocks_device(blocks, fs_orig: float, target_fs: float, fmin: float, fmax: float,
+                             outs=None, eps: float = 1e-10):
     """decimate_time, bandpass_filter, temporal_normalization("1bit") and
-    spectral_whitening of one (nt, nch) block (run_das_pipeline.py:165-168)
-    in one call that copies the block to the device once and leaves the result
-    there: -> (torch float32 CUDA tensor (nt_proc, nch), dt_proc, fs_proc),
-    bit-identical to the four functions in turn.  ``out`` (a CUDA float32
-    tensor of that shape) is reused when given."""
+    spectral_whitening (run_das_pipeline.py:165-168) of several (nt, nch)
+    blocks of one shape in one call: each block is copied to the device once
+    and its result stays there, and the zero-phase filters run the channels of
+    all the blocks side by side.  -> (list of torch float32 CUDA tensors
+    (nt_proc, nch), dt_proc, fs_proc); each block bit-identical to the four
+    functions in turn.  ``outs`` (CUDA float32 tensors of that shape) are
+    reused when given."""
     import torch
 
-    x = N.f32(data)
-    if x.ndim != 2:
-        raise ValueError("data must be (nt, n_channels)")
-    nt, nch = x.shape
+    xs = [N.f32(b) for b in blocks]
+    if not xs:
+        raise ValueError("no blocks")
+    if any(x.ndim != 2 for x in xs):
+        raise ValueError("blocks must be (nt, n_channels)")
+    nt, nch = xs[0].shape
+    if any(x.shape != (nt, nch) for x in xs):
+        raise ValueError("blocks of one call must have one shape")
     q = int(round(fs_orig / target_fs))
     lp = (None, None, 0, 0)
     if q > 1:
@@ -127,12 +133,34 @@ def preprocess_block_device(data: np.ndarray, fs_orig: float, target_fs: float, 
     nt1 = (nt + q - 1) // q
     bp = _sos_args(signal.butter(4, [fmin, fmax], btype="bandpass", fs=fs_proc, output="sos"),
                    nt1)
-    if out is None or tuple(out.shape) != (nt1, nch) or out.dtype != torch.float32 \
-            or not out.is_cuda or not out.is_contiguous():
-        out = torch.empty((nt1, nch), dtype=torch.float32, device="cuda")
+    outs = list(outs) if outs is not None else []
+    for i in range(len(xs)):
+        o = outs[i] if i < len(outs) else None
+        if o is None or tuple(o.shape) != (nt1, nch) or o.dtype != torch.float32 \
+                or not o.is_cuda or not o.is_contiguous():
+            o = torch.empty((nt1, nch), dtype=torch.float32, device="cuda")
+        if i < len(outs):
+            outs[i] = o
+        else:
+            outs.append(o)
+    outs = outs[:len(xs)]
+    xp = (N.P * len(xs))(*[N.ptr(x) for x in xs])
+    op = (N.P * len(xs))(*[o.data_ptr() for o in outs])
     ptr = lambda a: None if a is None else N.ptr(a)  # noqa: E731
-    N.check(N.lib().tsw_preprocess_device(
-        N.ptr(x), nt, nch, ptr(lp[0]), lp[2], ptr(lp[1]), lp[3], q, N.ptr(bp[0]), bp[2],
-        N.ptr(bp[1]), bp[3], float(fs_proc), float(fmin), float(fmax), float(eps), out.data_ptr(),
-        torch.cuda.current_stream(out.device).cuda_stream))
-    return out, 1.0 / fs_proc, fs_proc
+    N.check(N.lib().tsw_preprocess_device_blocks(
+        len(xs), xp, nt, nch, ptr(lp[0]), lp[2], ptr(lp[1]), lp[3], q, N.ptr(bp[0]), bp[2],
+        N.ptr(bp[1]), bp[3], float(fs_proc), float(fmin), float(fmax), float(eps), op,
+        torch.cuda.current_stream(outs[0].device).cuda_stream))
+    return outs, 1.0 / fs_proc, fs_proc
+
+
+def preprocess_block_device(data: np.ndarray, fs_orig: float, target_fs: float, fmin: float,
+                            fmax: float, out=None, eps: float = 1e-10):
+    """preprocess_blocks_device of one block -> (torch float32 CUDA tensor
+    (nt_proc, nch), dt_proc, fs_proc); ``out`` is reused when it fits."""
+    x = N.f32(data)
+    if x.ndim != 2:
+        raise ValueError("data must be (nt, n_channels)")
+    outs, dt, fs = preprocess_blocks_device([x], fs_orig, target_fs, fmin, fmax,
+                                            outs=None if out is None else [out], eps=eps)
+    return outs[0], dt, fs

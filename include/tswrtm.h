@@ -365,6 +365,16 @@ int tsw_preprocess_device(const float* x, int nt, int nch, const double* lp_sos,
                           const double* lp_zi, int lp_padlen, int q, const double* bp_sos,
                           int bp_nsec, const double* bp_zi, int bp_padlen, double fs_proc,
                           double fmin, double fmax, double eps, float* d_out, void* stream);
+/* The same for nblk blocks of one shape at once (x[b] on the host, d_out[b]
+ * on the device): the zero-phase filters run the channels of every block
+ * side by side in one launch (they are sequential in time, one thread per
+ * channel, so a single block leaves most of the GPU idle); each block's
+ * result equals tsw_preprocess_device's bit for bit. */
+int tsw_preprocess_device_blocks(int nblk, const float* const* x, int nt, int nch,
+                                 const double* lp_sos, int lp_nsec, const double* lp_zi,
+                                 int lp_padlen, int q, const double* bp_sos, int bp_nsec,
+                                 const double* bp_zi, int bp_padlen, double fs_proc, double fmin,
+                                 double fmax, double eps, float* const* d_out, void* stream);
 /* F-K fan filter of a gather g (nt, nx): 2-D FFT, keep wavenumbers k != 0 with
  * vmin <= |f| / |k| <= vmax, inverse FFT, real part (FP32 transforms). */
 int tsw_fk_filter(const float* g, int nt, int nx, double dt, double dx, double vmin, double vmax,

@@ -166,3 +166,39 @@ def test_stacked_correlations_equal_host_chain():
     np.testing.assert_array_equal(lags, want_lags)
     np.testing.assert_array_equal(ccfs, stack / np.float32(len(blocks)))
     assert np.abs(ccfs).max() > 0
+
+
+@pytest.mark.parametrize("fs_orig,target,nt,nch", [(100.0, 50.0, 3001, 37), (50.0, 50.0, 2000, 70)])
+def test_device_blocks_equal_single_blocks(fs_orig, target, nt, nch):
+    """Several blocks in one call (their channels filtered side by side, a
+    wave straddling two blocks when nch is not a multiple of 64) equal each
+    block preprocessed alone, bit for bit; reused outputs."""
+    from das_amd.preprocessing import preprocess_block_device, preprocess_blocks_device
+
+    xs = [_noise(nt, nch, seed=s, scale=3.0) for s in (11, 12, 13)]
+    want = [preprocess_block_device(x, fs_orig, target, 0.8, 4.5)[0].cpu().numpy() for x in xs]
+    outs = None
+    for _ in range(2):
+        outs, dt, fs = preprocess_blocks_device(xs, fs_orig, target, 0.8, 4.5, outs=outs)
+        assert len(outs) == 3
+        for o, w in zip(outs, want):
+            np.testing.assert_array_equal(o.cpu().numpy(), w)
+    with pytest.raises(ValueError, match="one shape"):
+        preprocess_blocks_device([xs[0], xs[1][:-1]], fs_orig, target, 0.8, 4.5)
+
+
+@pytest.mark.parametrize("group", [1, 2, 4])
+def test_stacked_correlations_any_group(group):
+    """The stack is the same whatever number of blocks are preprocessed per
+    call, including a last, smaller group and a change of block shape."""
+    from das_amd.pipeline import stacked_correlations
+
+    nch = 40
+    blocks = [_noise(4000, nch, seed=s) for s in (4, 5, 6)] + [_noise(3000, nch, seed=7)]
+    ref, lags, dtp = stacked_correlations(blocks, 100.0, 50.0, 0.8, 4.5, [3, -2],
+                                          np.arange(0, nch, 3), 2.0, group=1)
+    got, lags2, dtp2 = stacked_correlations(blocks, 100.0, 50.0, 0.8, 4.5, [3, -2],
+                                            np.arange(0, nch, 3), 2.0, group=group)
+    np.testing.assert_array_equal(got, ref)
+    np.testing.assert_array_equal(lags2, lags)
+    assert dtp2 == dtp
