@@ -133,17 +133,15 @@ def preprocess_blocks_device(blocks, fs_orig: float, target_fs: float, fmin: flo
     nt1 = (nt + q - 1) // q
     bp = _sos_args(signal.butter(4, [fmin, fmax], btype="bandpass", fs=fs_proc, output="sos"),
                    nt1)
-    outs = list(outs) if outs is not None else []
-    for i in range(len(xs)):
-        o = outs[i] if i < len(outs) else None
-        if o is None or tuple(o.shape) != (nt1, nch) or o.dtype != torch.float32 \
-                or not o.is_cuda or not o.is_contiguous():
-            o = torch.empty((nt1, nch), dtype=torch.float32, device="cuda")
-        if i < len(outs):
-            outs[i] = o
-        else:
-            outs.append(o)
-    outs = outs[:len(xs)]
+    given = list(outs) if outs is not None else []
+
+    def fits(o):
+        return (o is not None and tuple(o.shape) == (nt1, nch) and o.dtype == torch.float32
+                and o.is_cuda and o.is_contiguous())
+
+    outs = [given[i] if i < len(given) and fits(given[i])
+            else torch.empty((nt1, nch), dtype=torch.float32, device="cuda")
+            for i in range(len(xs))]
     xp = (N.P * len(xs))(*[N.ptr(x) for x in xs])
     op = (N.P * len(xs))(*[o.data_ptr() for o in outs])
     ptr = lambda a: None if a is None else N.ptr(a)  # noqa: E731
