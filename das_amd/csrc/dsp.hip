@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <thread>
 #include <vector>
 
 #include "engine.h"
@@ -261,6 +262,11 @@ static unsigned gridn(long long n) {
 struct DspCache {
   std::map<std::array<int, 3>, hipfftHandle> plans;  // (kind, n0, n1) -> plan
   DevBuf buf[4];
+  // two pinned staging buffers for host blocks, and the event of the copy
+  // that last read each
+  void* pin[2] = {nullptr, nullptr};
+  size_t pin_bytes = 0;
+  hipEvent_t pin_done[2] = {nullptr, nullptr};
 };
 
 static DspCache& dsp_cache() {
@@ -297,6 +303,45 @@ static int cached_plan(int kind, int n0, int n1, hipfftHandle* out) {
 }
 
 static DevBuf& scratch(int i) { return dsp_cache().buf[i]; }
+
+// Copies nbytes from host memory (pageable) to d_dst on st through the pinned
+// staging buffer `slot`: the host copy is split over several threads, and it
+// waits only for the earlier copy that read the same buffer, so the host copy
+// of one block overlaps the transfer of the one before.
+static int staged_h2d(void* d_dst, const void* src, size_t nbytes, int slot, hipStream_t st) {
+  DspCache& c = dsp_cache();
+  if (c.pin_bytes < nbytes) {
+    for (int i = 0; i < 2; ++i) {
+      if (c.pin_done[i]) TSW_HIP(hipEventSynchronize(c.pin_done[i]));
+      if (c.pin[i]) TSW_HIP(hipHostFree(c.pin[i]));
+      c.pin[i] = nullptr;
+    }
+    c.pin_bytes = 0;
+    for (int i = 0; i < 2; ++i)
+      if (hipHostMalloc(&c.pin[i], nbytes, hipHostMallocDefault) != hipSuccess) {
+        set_error("pinned staging buffer of %zu bytes: allocation failed", nbytes);
+        return kErrMem;
+      }
+    c.pin_bytes = nbytes;
+  }
+  if (!c.pin_done[slot]) TSW_HIP(hipEventCreateWithFlags(&c.pin_done[slot], hipEventDisableTiming));
+  TSW_HIP(hipEventSynchronize(c.pin_done[slot]));
+  char* dst = static_cast<char*>(c.pin[slot]);
+  const char* from = static_cast<const char*>(src);
+  const int nth = nbytes >= (64u << 20) ? 8 : 1;
+  const size_t part = (nbytes + nth - 1) / nth;
+  std::vector<std::thread> th;
+  for (int t = 1; t < nth; ++t) {
+    const size_t o = t * part;
+    if (o < nbytes)
+      th.emplace_back([=] { std::memcpy(dst + o, from + o, std::min(part, nbytes - o)); });
+  }
+  std::memcpy(dst, from, std::min(part, nbytes));
+  for (auto& t : th) t.join();
+  TSW_HIP(hipMemcpyAsync(d_dst, dst, nbytes, hipMemcpyHostToDevice, st));
+  TSW_HIP(hipEventRecord(c.pin_done[slot], st));
+  return kOk;
+}
 
 static int sos_check(int nt, int nch, const double* sos, int nsec, int padlen, int step) {
   TSW_ARG(sos != nullptr, "null argument");
@@ -464,8 +509,7 @@ int tsw_preprocess_device_blocks(int nblk, const float* const* x, int nt, int nc
   TSW_TRY(a.ensure((size_t)(xs * nblk) * sizeof(float)));
   TSW_TRY(b.ensure((size_t)(os * nblk) * sizeof(float)));
   for (int k = 0; k < nblk; ++k)
-    TSW_HIP(hipMemcpyAsync(a.f() + k * xs, x[k], (size_t)xs * sizeof(float),
-                           hipMemcpyHostToDevice, st));
+    TSW_TRY(staged_h2d(a.f() + k * xs, x[k], (size_t)xs * sizeof(float), k & 1, st));
   float* cur = a.f();
   float* other = b.f();
   long long cs = xs;
