@@ -229,11 +229,18 @@ static unsigned grid1(long long n) {
   return (unsigned)std::max<long long>(1, std::min<long long>(b, 65536));
 }
 
+struct XcorrScratch {
+  DevBuf chs, S, R, pairs, S64;
+};
+
 static int xcorr_device(const float* d_data, int nt, int nch, const int* src_ch, int nsrc,
                         const int* rec_ch, int nrec, int nlag, int accumulate, float* d_out,
                         hipStream_t st) {
   const int nl = 2 * nlag + 1;
-  DevBuf chs, S, R, pairs;
+  // scratch kept across calls (grown as needed, held until the process ends);
+  // the entry points hold the API lock and drain the stream before returning
+  static XcorrScratch* scr = new XcorrScratch();
+  DevBuf &chs = scr->chs, &S = scr->S, &R = scr->R, &pairs = scr->pairs;
   std::vector<int> h(src_ch, src_ch + nsrc);
   h.insert(h.end(), rec_ch, rec_ch + nrec);
   TSW_TRY(chs.ensure(h.size() * sizeof(int)));
@@ -254,7 +261,7 @@ static int xcorr_device(const float* d_data, int nt, int nch, const int* src_ch,
   // the source rows in FP64, padded with zeros so that the last chunk (it
   // starts below nt and spans at most kXC samples) stays inside its row
   const int ntp = nt + kXC;
-  DevBuf S64;
+  DevBuf& S64 = scr->S64;
   TSW_TRY(S64.ensure((size_t)nsrc * ntp * sizeof(double)));
   hipLaunchKernelGGL(k_rows_f64, dim3(grid1((long long)nsrc * ntp)), dim3(256), 0, st, S.f(), nt,
                      ntp, nsrc, (double*)S64.p);
@@ -271,7 +278,7 @@ static int xcorr_device(const float* d_data, int nt, int nch, const int* src_ch,
   hipLaunchKernelGGL(k_lag_major, dim3((nl + 31) / 32, (nrec + 31) / 32, nsrc), dim3(32, 8), 0,
                      st, pairs.f(), nrec, nl, d_out, accumulate);
   TSW_HIP(hipGetLastError());
-  TSW_HIP(hipStreamSynchronize(st));  // the scratch buffers are freed on return
+  TSW_HIP(hipStreamSynchronize(st));  // the scratch buffers are free for the next call
   return kOk;
 }
 
@@ -326,6 +333,7 @@ int tsw_xcorr(const float* data, int nt, int nch, const int* src_ch, int nsrc, c
 int tsw_xcorr_device(const float* d_data, int nt, int nch, const int* src_ch, int nsrc,
                      const int* rec_ch, int nrec, int nlag, int accumulate, float* d_out,
                      void* stream) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
   TSW_ARG(d_data && d_out, "null argument");
   TSW_TRY(xcorr_args(nt, nch, src_ch, nsrc, rec_ch, nrec, nlag));
   Ctx* ctx;
