@@ -416,19 +416,20 @@ def preprocessing_timing(log):
     t2 = time.perf_counter()
     out["device_chain_ms"] = round((t1 - t0) * 1e3, 2)
     out["device_chain_plus_correlation_ms"] = round((t2 - t0) * 1e3, 2)
-    # four blocks per call, as stacked_correlations groups them
+    # several blocks per call, as stacked_correlations groups them (8 by default)
     from das_amd.preprocessing import preprocess_blocks_device
 
-    ws, _, _ = preprocess_blocks_device([x] * 4, 100.0, 50.0, 0.8, 4.5)
+    group = 8
+    ws, _, _ = preprocess_blocks_device([x] * group, 100.0, 50.0, 0.8, 4.5)
     torch.cuda.synchronize()
     t3 = time.perf_counter()
-    preprocess_blocks_device([x] * 4, 100.0, 50.0, 0.8, 4.5, outs=ws)
+    preprocess_blocks_device([x] * group, 100.0, 50.0, 0.8, 4.5, outs=ws)
     torch.cuda.synchronize()
-    out["device_chain_4_blocks_per_call_ms_per_block"] = round(
-        (time.perf_counter() - t3) * 1e3 / 4, 2)
+    out["device_chain_blocks_per_call"] = group
+    out["device_chain_grouped_ms_per_block"] = round((time.perf_counter() - t3) * 1e3 / group, 2)
     log(f"preprocessing on the device: {(t1 - t0) * 1e3:.1f} ms per block, "
         f"{(t2 - t0) * 1e3:.1f} ms with its correlations into the stack, "
-        f"{out['device_chain_4_blocks_per_call_ms_per_block']:.1f} ms per block four at a time")
+        f"{out['device_chain_grouped_ms_per_block']:.1f} ms per block {group} at a time")
     return out
 
 

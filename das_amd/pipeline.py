@@ -28,7 +28,7 @@ from .wave_solver import ricker_wavelet
 def stacked_correlations(blocks: Iterable[np.ndarray], fs_orig: float, target_fs: float,
                          fmin: float, fmax: float, source_channels: Sequence[int],
                          receiver_channels: np.ndarray, max_lag_s: float,
-                         group: int = 4) -> Tuple[np.ndarray, np.ndarray, float]:
+                         group: int = 8) -> Tuple[np.ndarray, np.ndarray, float]:
     """run_das_pipeline.py:160-213 and :232-235 -> (ccfs (n_src, 2 n_lags + 1,
     n_rec) float32 averaged over the blocks, lags, dt_proc).  Up to ``group``
     consecutive blocks of one shape are preprocessed in one call (their
