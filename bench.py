@@ -51,11 +51,44 @@ def cpu_baseline(cfg, budget_cell_updates):
         if done >= budget_cell_updates:
             break
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(done / t / 1e9, 4), "unit": "Gcell-updates/s", "cores": cores,
-            "kind": kind,
-            "sample": (f"{len(used)} of 64 shots: " + "; ".join(used) +
-                       f"; {done / 1e9:.2f} G cell-updates in {t:.1f} s, per-step kernels "
-                       "and snapshots in host RAM as the reference")}
+    out = {"value": round(done / t / 1e9, 4), "unit": "Gcell-updates/s", "cores": cores,
+           "kind": kind,
+           "sample": (f"{len(used)} of 64 shots: " + "; ".join(used) +
+                      f"; {done / 1e9:.2f} G cell-updates in {t:.1f} s, per-step kernels "
+                      "and snapshots in host RAM as the reference")}
+    out.update(cpu_reference_points(kind))
+    return out
+
+
+def cpu_reference_points(kind):
+    """The same CPU engine on the single-shot workloads: the 200 x 400 acoustic
+    shot (nt 4000, k 2) migrated in full, and the forward of the 400 x 1600
+    elastic shot over its first 800 of 8000 steps (the oracle's restatement:
+    the reference elastic kernel reads a row above the grid)."""
+    import numpy as np
+
+    from das_amd.workloads import PlainAcousticShot, PlainElasticShot
+    from oracle import oracle as ora
+
+    c0 = PlainAcousticShot()
+    g0 = ora.Grid(c0.nx, c0.nz, c0.dx, c0.dx, c0.dt, c0.n_sponge)
+    data0 = (np.random.default_rng(0).standard_normal((c0.nt, len(c0.rix))) * 1e-3
+             ).astype(np.float32)
+    t0 = time.perf_counter()
+    ora.migrate_shot(g0, c0.mig_vel, c0.mig_vel, c0.src, c0.six, c0.siz, data0, c0.rix, c0.riz,
+                     c0.k, use_reference_kernels=(kind == "reference"))
+    t_c0 = time.perf_counter() - t0
+    c1 = PlainElasticShot(nt=800)
+    g1 = ora.Grid(c1.nx, c1.nz, c1.dx, c1.dx, c1.dt, c1.n_sponge)
+    t0 = time.perf_counter()
+    ora.elastic_forward(g1, c1.model_true, c1.src, c1.six, c1.siz, c1.rix, c1.riz)
+    t_c1 = time.perf_counter() - t0
+    log(f"cpu: acoustic 200x400 shot {t_c0:.2f}s, elastic forward 800 steps {t_c1:.2f}s")
+    return {"acoustic_shot_200x400": {"s": round(t_c0, 3), "kind": kind, "gcell_updates_per_s":
+                                      round(2 * c0.nt * c0.nx * c0.nz / t_c0 / 1e9, 4)},
+            "elastic_forward_400x1600": {"steps": c1.nt, "s": round(t_c1, 3), "kind": "port",
+                                         "gcell_updates_per_s":
+                                             round(c1.nt * c1.nx * c1.nz / t_c1 / 1e9, 4)}}
 
 
 ELASTIC_BYTES_PER_CELL_UPDATE = 60.0  # 5 fields read + written, 5 parameters read
