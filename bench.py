@@ -60,6 +60,34 @@ def cpu_baseline(cfg, budget_cell_updates):
     return out
 
 
+def plain_shot_200x400_ms():
+    """The 200 x 400 acoustic shot (nt 4000, k 2) migrated by the plain GPU
+    path (tsw_migrate_shot: one launch per time step, snapshots in HBM), the
+    second of two calls, in ms: the GPU side of the CPU baseline's
+    acoustic_shot_200x400."""
+    import numpy as np
+    import torch
+
+    from das_amd.rtm_engine import TSWRTMEngine
+    from das_amd.velocity_flood import FloodModels
+    from das_amd.wave_solver import WaveSolver2D
+    from das_amd.workloads import PlainAcousticShot
+
+    c0 = PlainAcousticShot()
+    solver = WaveSolver2D(c0.nx, c0.nz, c0.dx, c0.dx, c0.dt, n_sponge=c0.n_sponge)
+    data0 = (np.random.default_rng(0).standard_normal((c0.nt, len(c0.rix))) * 1e-3
+             ).astype(np.float32)
+    fm = FloodModels(c0.mig_vel, c0.mig_vel, c0.mig_vel, 0, 0)
+    eng = TSWRTMEngine(solver)
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.migrate_shot(fm, c0.src, c0.six, c0.siz, data0, c0.rix, c0.riz, c0.k)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+    return round(el * 1e3, 2)
+
+
 def cpu_reference_points(kind):
     """The same CPU engine on the single-shot workloads: the 200 x 400 acoustic
     shot (nt 4000, k 2) migrated in full, and the forward of the 400 x 1600
@@ -588,6 +616,7 @@ def main():
                             "shots_per_s": round(cfg.n_shots / tp, 3),
                             "mode": "one shot at a time, one launch per time step"}
         out["speedup_vs_plain"] = round(shots_per_s / (cfg.n_shots / tp), 2)
+        out["plain_gpu"]["acoustic_shot_200x400_ms"] = plain_shot_200x400_ms()
     batch.close()
     if not args.no_elastic:
         out["elastic"] = elastic_leg(args, log)
