@@ -657,7 +657,8 @@ __attribute__((amdgpu_waves_per_eu(kTbWavesPerEu<R, WX, WZ>))) void k_fwi_tb(Fwi
       lzz = lzz + TSW_C1 * (zm1 + zp1);
       lzz = (lzz + TSW_C0 * pc) * a.rdz2;
       const float lap = lxx + lzz;
-      if (sv && (m_core >> q & 1u)) sv[(row0 + q) * nx + col] = (m_int >> q & 1u) ? lap : 0.0f;
+      if (sv && (m_core >> q & 1u))  // read once, by the adjoint sweep
+        __builtin_nontemporal_store((m_int >> q & 1u) ? lap : 0.0f, sv + (row0 + q) * nx + col);
       float v = (2.0f * pc - u1[q]) + cf[q] * lap;
       v = v * sp[q];
       // cells the step never writes keep the level three buffers back
@@ -835,7 +836,7 @@ __attribute__((amdgpu_waves_per_eu(kTbWavesPerEu<R, WX, WZ>))) void k_fwi_adj_tb
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const unsigned idx = (unsigned)((row0 + q) * nx + col);
-      lun[q] = ((m_core & m_int) >> q & 1u) ? lu[idx] : 0.0f;
+      lun[q] = ((m_core & m_int) >> q & 1u) ? __builtin_nontemporal_load(lu + idx) : 0.0f;
     }
     float nv[R];
     const float* xr = X + 2;  // column offset of lx = 0
