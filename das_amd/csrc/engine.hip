@@ -688,7 +688,11 @@ static int ref_acoustic(const float* p_curr, const float* p_prev, float* p_next,
   s.riz = -1;
   s.rdata_it = -1;
   s.rec_it = -1;
-  if (nx >= 5 && nz >= 3) launch_acoustic_step(s, st);
+  // nx < 5 leaves no interior column, but the free-surface rows are still
+  // written for every ix (wave2d.c:75-81): the step kernel's row-2 cells do
+  // that.  With nz < 3 the reference's row 1 = -row 2 reads past the grid;
+  // nothing is written then.
+  if (nz >= 3) launch_acoustic_step(s, st);
   TSW_HIP(hipGetLastError());
   TSW_HIP(hipMemcpyAsync(p_next, g.buf[2].p, nb, hipMemcpyDeviceToHost, st));
   TSW_HIP(hipStreamSynchronize(st));

@@ -13,6 +13,12 @@
 namespace tsw {
 
 static void put(std::string& out, const char* fmt, double v) {
+  // Python formats every NaN as "nan", whatever its sign bit; printf would
+  // write "-nan" for a NaN whose sign bit is set
+  if (v != v) {
+    out += "nan";
+    return;
+  }
   char buf[64];
   const int n = std::snprintf(buf, sizeof buf, fmt, v);
   out.append(buf, (size_t)n);

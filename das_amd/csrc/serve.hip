@@ -13,6 +13,7 @@
 // Every request runs through the same device functions as the single-shot
 // paths, with no cross-request arithmetic, so its result is bit-identical
 // alone and in any mix.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -46,6 +47,7 @@ struct Req {
   long long need = 0;     // device bytes while running
   long long keep = 0;     // device bytes kept after completion (image + illumination)
   int slot = -1;
+  bool in_queue = false;  // listed in Server::queue
   bool resident = false;  // models are the window [z0, z0+nz) x [x0, x0+nx) of the server's
   int x0 = 0, z0 = 0;     //   resident models, copied on the device at admission
   int cursor = 0;         // acoustic: next block; elastic: unused (global step based)
@@ -369,7 +371,17 @@ static int tick(Server* S, std::vector<Req*>& done, long long& cu, double& busy)
   return kOk;
 }
 
+// Host copies of a request's inputs (freed at admission, cancellation or
+// failure: the request no longer needs them).
+static void drop_host_inputs(Req* r) {
+  for (auto& v : r->model) std::vector<float>().swap(v);
+  std::vector<float>().swap(r->src);
+  std::vector<float>().swap(r->data);
+  std::vector<int>().swap(r->rix);
+}
+
 static void fail_request(Server* S, Req* r, int state, const char* err) {
+  drop_host_inputs(r);
   r->state = state;
   if (err) r->err = err;
   r->t_done = S->now();
@@ -392,6 +404,7 @@ static void worker_loop(Server* S) {
       Req* r = S->queue.front();
       if (r->cancel) {
         S->queue.pop_front();
+        r->in_queue = false;
         continue;
       }
       int s = -1;
@@ -408,6 +421,7 @@ static void worker_loop(Server* S) {
         break;
       }
       S->queue.pop_front();
+      r->in_queue = false;
       r->heavy = h;
       r->keepb = kb;
       r->slot = s;
@@ -635,6 +649,7 @@ int tsw_server_submit(tsw_server_t h, const tsw_shot_desc* d, long long* id_out)
   r->t_submit = S->now();
   *id_out = r->id;
   S->queue.push_back(r.get());
+  r->in_queue = true;
   S->reqs[r->id] = std::move(r);
   S->cv.notify_all();
   return kOk;
@@ -683,8 +698,16 @@ int tsw_server_result(tsw_server_t h, long long id, float* image_out, float* ill
   auto it = S->reqs.find(id);
   TSW_ARG(it != S->reqs.end(), "unknown request %lld", id);
   Req& r = *it->second;
+  // a failed or cancelled request is forgotten once this call reports it
+  // (it holds no device memory and is in neither the queue nor a slot)
   if (r.state == kFailed) {
     set_error("request %lld failed: %s", id, r.err.c_str());
+    S->reqs.erase(it);
+    return kErrState;
+  }
+  if (r.state == kCancelled && !r.in_queue && r.slot < 0) {
+    set_error("request %lld was cancelled", id);
+    S->reqs.erase(it);
     return kErrState;
   }
   TSW_ARG(r.state == kDone, "request %lld is not done (state %d)", id, r.state);
@@ -708,6 +731,12 @@ int tsw_server_cancel(tsw_server_t h, long long id) {
     r.cancel = true;
     r.state = kCancelled;
     S->cancelled++;
+    // out of the queue and without its host copies at once, so a cancelled
+    // request holds only its record until tsw_server_result reports it
+    auto q = std::find(S->queue.begin(), S->queue.end(), &r);
+    if (q != S->queue.end()) S->queue.erase(q);
+    r.in_queue = false;
+    drop_host_inputs(&r);
   } else if (r.state == kRunning) {
     r.cancel = true;  // takes effect at the next tick boundary
   }

@@ -68,7 +68,9 @@ typedef struct tsw_shot_desc {
 
 /* ------------------------------------------------------------------ 1. */
 
-/* Replaces acoustic_2d_step, tsw_rtm/c_engine/wave2d.c:19-31. */
+/* Replaces acoustic_2d_step, tsw_rtm/c_engine/wave2d.c:19-31.  With
+ * nz < 3 and a free surface the reference's row 1 = -row 2 (wave2d.c:79)
+ * reads past the grid; here p_next is then left unchanged. */
 void acoustic_2d_step(const float* p_curr, const float* p_prev, float* p_next, const float* vel,
                       const float* sponge, int nx, int nz, float dx, float dz, float dt,
                       int free_surface_top);
@@ -242,9 +244,14 @@ int tsw_server_poll(tsw_server_t server, long long id, int* state, double* t_sub
                     double* t_admit, double* t_done);
 /* Blocks until request `id` is done, failed or cancelled (or the timeout). */
 int tsw_server_wait(tsw_server_t server, long long id, double timeout_s);
-/* Copies a done request's image and illumination and forgets the request. */
+/* Copies a done request's image and illumination and forgets the request.
+ * For a failed or cancelled request it returns kErrState with the reason in
+ * tsw_last_error() and forgets the request as well (a cancelled request that
+ * was running is forgotten once its slot has been released at the next tick
+ * boundary; until then the call reports "not done"). */
 int tsw_server_result(tsw_server_t server, long long id, float* image_out, float* illum_out);
-/* Cancels a queued request, or a running one at the next tick boundary. */
+/* Cancels a queued request (it leaves the queue and frees its host copies at
+ * once), or a running one at the next tick boundary. */
 int tsw_server_cancel(tsw_server_t server, long long id);
 int tsw_server_stats(tsw_server_t server, long long* completed, long long* failed,
                      long long* cancelled, long long* queued, long long* running,

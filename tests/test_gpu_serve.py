@@ -142,8 +142,14 @@ def test_budget_rejection_cancel_and_injected_failure(reqs):
     np.testing.assert_array_equal(got[1], expect[1][1])
     with pytest.raises(N.TswError):
         srv.result(c)
+    # reported once, then forgotten: failed and cancelled records do not pile up
+    with pytest.raises(N.TswError, match="cancelled"):
+        srv.result(b)
+    for rid in (b, c):
+        with pytest.raises(ValueError, match="unknown request"):
+            srv.poll(rid)
     st = srv.stats()
-    assert st["failed"] == 1 and st["cancelled"] == 1
+    assert st["failed"] == 1 and st["cancelled"] == 1 and st["mem_used"] == 0
     srv.close()
 
 

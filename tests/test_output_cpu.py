@@ -30,6 +30,25 @@ def test_csv_bytes_equal_reference_format(tmp_path, k):
     assert (tmp_path / "a.csv").read_bytes() == (tmp_path / "b.csv").read_bytes()
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_csv_nan_sign_bit(tmp_path, dtype):
+    """A NaN with its sign bit set prints "nan", as Python's format does
+    (visualizer.py:36-38), in the matrix and in both axes."""
+    from das_amd.output import save_matrix_csv
+
+    nneg = np.copysign(np.nan, -1.0)
+    m = np.array([[nneg, np.nan, -np.inf], [1.5, nneg, 0.0]], dtype=dtype)
+    assert np.signbit(m[0, 0]) and np.signbit(m[1, 1])
+    ra = np.array([nneg, 1.0])
+    ca = np.array([0.0, nneg, np.nan])
+    save_matrix_csv(tmp_path / "a.csv", m, ra, ca, "z", "x")
+    OO.save_matrix_csv(tmp_path / "b.csv", m, ra, ca, "z", "x")
+    a = (tmp_path / "a.csv").read_bytes()
+    assert a == (tmp_path / "b.csv").read_bytes()
+    assert b"-nan" not in a
+    assert a.splitlines()[1].startswith(b"nan,nan,nan,-inf")
+
+
 def test_oracle_round_trip(tmp_path):
     m = np.random.default_rng(1).standard_normal((4, 6)).astype(np.float32)
     OO.save_matrix_csv(tmp_path / "o.csv", m, np.arange(4), np.arange(6) * 0.5, "t", "x")
