@@ -30,13 +30,33 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cfg, budget_cell_updates):
-    """The reference engine (wave2d.c compiled from source, oracle/_ref) driven
-    by the oracle's C time loop, on the first shots of the same workload."""
-    import numpy as np
+def physical_cores():
+    """Physical cores among the CPUs this process may run on (one per
+    (package, core id) pair, so SMT siblings count once) -> (cores, cpus)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    seen = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            with open(base + "core_id") as f:
+                core = f.read().strip()
+            with open(base + "physical_package_id") as f:
+                pkg = f.read().strip()
+        except OSError:
+            core, pkg = str(c), "0"
+        seen.add((pkg, core))
+    return len(seen), len(cpus)
 
+
+def cpu_worker(threads, budget_cell_updates, full):
+    """Child-process side of the CPU baseline (no GPU use): the reference
+    engine (wave2d.c compiled from source, oracle/_ref) driven by the oracle's
+    C time loop on the first shots of the batched workload, with OpenMP
+    placement fixed by the parent's environment; prints one JSON line."""
+    from das_amd.workloads import BatchedWindows
     from oracle import oracle as ora
 
+    cfg = BatchedWindows(n_shots=64).build(with_data=True)
     kind = "reference" if ora.ref_lib() is not None else "port"
     done, t, used = 0, 0.0, []
     for i, s in enumerate(cfg.shots):
@@ -47,31 +67,65 @@ def cpu_baseline(cfg, budget_cell_updates):
         t += time.perf_counter() - t0
         done += 2 * s.nt * s.nx * s.nz
         used.append(f"shot {i} ({s.nz}x{s.nx}, nt={s.nt})")
-        log(f"cpu baseline: {used[-1]} {t:.1f}s")
+        log(f"cpu baseline ({threads} threads): {used[-1]} {t:.1f}s")
         if done >= budget_cell_updates:
             break
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    out = {"value": round(done / t / 1e9, 4), "unit": "Gcell-updates/s", "cores": cores,
+    out = {"value": round(done / t / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
            "kind": kind,
            "sample": (f"{len(used)} of 64 shots: " + "; ".join(used) +
                       f"; {done / 1e9:.2f} G cell-updates in {t:.1f} s, per-step kernels "
                       "and snapshots in host RAM as the reference")}
-    out.update(cpu_reference_points(kind))
+    out.update(cpu_reference_points(kind, elastic=full))
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(budget_cell_updates, budget_one_core):
+    """The CPU baseline on the box's host cores: once with one OpenMP thread
+    per physical core (OMP_PROC_BIND=close, OMP_PLACES=cores) and once on one
+    core, each in a child process so that the placement applies to the
+    reference engine's OpenMP team from its start."""
+    import subprocess
+
+    cores, cpus = physical_cores()
+    runs = {}
+    for threads, budget, full in ((cores, budget_cell_updates, True),
+                                  (1, budget_one_core, False)):
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+                   OMP_PLACES="cores")
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", str(threads),
+               "--cpu-budget", str(budget / 1e9)] + ([] if full else ["--cpu-worker-lite"])
+        r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"CPU baseline worker ({threads} threads) failed: {r.returncode}")
+        runs[threads] = json.loads(r.stdout.strip().splitlines()[-1])
+    out = runs[cores]
+    out["placement"] = (f"OMP_NUM_THREADS={cores} (physical cores of the {cpus} CPUs this "
+                        "process may use), OMP_PROC_BIND=close, OMP_PLACES=cores")
+    one = runs[1]
+    out["one_core"] = {"value": one["value"], "unit": one["unit"], "cores": 1,
+                       "sample": one["sample"],
+                       "acoustic_shot_200x400": one["acoustic_shot_200x400"]}
+    out["speedup_vs_one_core"] = round(out["value"] / one["value"], 2)
     return out
 
 
-def plain_shot_200x400_ms():
-    """The 200 x 400 acoustic shot (nt 4000, k 2) migrated by the plain GPU
-    path (tsw_migrate_shot: one launch per time step, snapshots in HBM), the
-    second of two calls, in ms: the GPU side of the CPU baseline's
-    acoustic_shot_200x400."""
+def plain_shot_200x400():
+    """The 200 x 400 acoustic shot (nt 4000, k 2), the GPU side of the CPU
+    baseline's acoustic_shot_200x400, on two GPU paths, in ms (second of two
+    runs each):
+      * ``migrate_shot_ms``: TSWRTMEngine.migrate_shot from host arrays
+        (tsw_migrate_shot, copies included), which runs the shot through the
+        temporally blocked kernel as a one-shot batch;
+      * ``per_step_ms``: the same shot resident, one launch of the per-step
+        kernel per time step (no graph)."""
     import numpy as np
     import torch
 
+    from das_amd.batch import ShotBatch
     from das_amd.rtm_engine import TSWRTMEngine
     from das_amd.velocity_flood import FloodModels
     from das_amd.wave_solver import WaveSolver2D
-    from das_amd.workloads import PlainAcousticShot
+    from das_amd.workloads import PlainAcousticShot, Shot
 
     c0 = PlainAcousticShot()
     solver = WaveSolver2D(c0.nx, c0.nz, c0.dx, c0.dx, c0.dt, n_sponge=c0.n_sponge)
@@ -85,14 +139,50 @@ def plain_shot_200x400_ms():
         eng.migrate_shot(fm, c0.src, c0.six, c0.siz, data0, c0.rix, c0.riz, c0.k)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-    return round(el * 1e3, 2)
+    shot = Shot(nx=c0.nx, nz=c0.nz, nt=c0.nt, vel_fwd=c0.mig_vel, vel_bwd=c0.mig_vel,
+                src=c0.src, six=c0.six, siz=c0.siz, data=data0, rix=c0.rix, riz=c0.riz)
+    b = ShotBatch([shot], c0.dx, c0.dx, c0.dt, n_sponge=c0.n_sponge, k=c0.k, slots=1,
+                  graph_steps=0, kernel=0)
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.run("plain")
+        torch.cuda.synchronize()
+        ep = time.perf_counter() - t0
+    b.close()
+    return {"migrate_shot_ms": round(el * 1e3, 2), "per_step_ms": round(ep * 1e3, 2),
+            "note": "migrate_shot: host arrays in and out, blocked kernel as a one-shot batch; "
+                    "per_step: resident, one per-step kernel launch per time step"}
 
 
-def cpu_reference_points(kind):
+def check_against_per_step(cfg, batch, shots, log):
+    """Shots of the timed batch against the same shots migrated alone with the
+    per-step kernel (one launch per time step): bit-for-bit equality."""
+    from das_amd.batch import ShotBatch
+
+    out = {}
+    for i in shots:
+        s = cfg.shots[i]
+        b = ShotBatch([s], cfg.dx, cfg.dx, cfg.dt, n_sponge=cfg.n_sponge, k=cfg.k, slots=1,
+                      graph_steps=100, kernel=0)
+        b.run("batched")
+        ref = b.shot_result(0)
+        b.close()
+        got = batch.shot_result(i)
+        same = bool(np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]))
+        diff = float(np.linalg.norm(got[0].astype(np.float64) - ref[0]) /
+                     max(np.linalg.norm(ref[0].astype(np.float64)), 1e-300))
+        out[f"shot_{i}"] = {"grid": f"{s.nz}x{s.nx}", "nt": s.nt, "bit_equal": same,
+                            "image_rel_l2": diff}
+        log(f"check: shot {i} ({s.nz}x{s.nx}, nt {s.nt}) bit-equal to per-step: {same}")
+    return out
+
+
+def cpu_reference_points(kind, elastic=True):
     """The same CPU engine on the single-shot workloads: the 200 x 400 acoustic
-    shot (nt 4000, k 2) migrated in full, and the forward of the 400 x 1600
-    elastic shot over its first 800 of 8000 steps (the oracle's restatement:
-    the reference elastic kernel reads a row above the grid)."""
+    shot (nt 4000, k 2) migrated in full, and (``elastic``) the forward of the
+    400 x 1600 elastic shot over its first 800 of 8000 steps (the oracle's
+    restatement: the reference elastic kernel reads a row above the grid)."""
     import numpy as np
 
     from das_amd.workloads import PlainAcousticShot, PlainElasticShot
@@ -106,17 +196,20 @@ def cpu_reference_points(kind):
     ora.migrate_shot(g0, c0.mig_vel, c0.mig_vel, c0.src, c0.six, c0.siz, data0, c0.rix, c0.riz,
                      c0.k, use_reference_kernels=(kind == "reference"))
     t_c0 = time.perf_counter() - t0
+    pt0 = {"acoustic_shot_200x400": {"s": round(t_c0, 3), "kind": kind, "gcell_updates_per_s":
+                                     round(2 * c0.nt * c0.nx * c0.nz / t_c0 / 1e9, 4)}}
+    if not elastic:
+        return pt0
     c1 = PlainElasticShot(nt=800)
     g1 = ora.Grid(c1.nx, c1.nz, c1.dx, c1.dx, c1.dt, c1.n_sponge)
     t0 = time.perf_counter()
     ora.elastic_forward(g1, c1.model_true, c1.src, c1.six, c1.siz, c1.rix, c1.riz)
     t_c1 = time.perf_counter() - t0
     log(f"cpu: acoustic 200x400 shot {t_c0:.2f}s, elastic forward 800 steps {t_c1:.2f}s")
-    return {"acoustic_shot_200x400": {"s": round(t_c0, 3), "kind": kind, "gcell_updates_per_s":
-                                      round(2 * c0.nt * c0.nx * c0.nz / t_c0 / 1e9, 4)},
-            "elastic_forward_400x1600": {"steps": c1.nt, "s": round(t_c1, 3), "kind": "port",
-                                         "gcell_updates_per_s":
-                                             round(c1.nt * c1.nx * c1.nz / t_c1 / 1e9, 4)}}
+    pt0["elastic_forward_400x1600"] = {"steps": c1.nt, "s": round(t_c1, 3), "kind": "port",
+                                       "gcell_updates_per_s":
+                                           round(c1.nt * c1.nx * c1.nz / t_c1 / 1e9, 4)}
+    return pt0
 
 
 ELASTIC_BYTES_PER_CELL_UPDATE = 60.0  # 5 fields read + written, 5 parameters read
@@ -494,14 +587,34 @@ def preprocessing_timing(log):
     return out
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the step kernel from the committed rocprofv3
-    --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
-    p = os.path.join(ROOT, "profiles", "batch_step_traffic.json")
+def blocked_algorithmic_bytes(shots, T, k):
+    """Compulsory HBM bytes of one run of the temporally blocked scheme (T
+    steps per launch): per shot, phase and block of T steps each cell reads
+    the two carried levels, the model coefficient and its image/illumination
+    accumulator (16 B) and writes the two levels and the accumulator (12 B);
+    the first block of a phase reads only the coefficient, the last writes
+    only the accumulator; plus 4 B per cell per snapshot level, written by the
+    forward phase and read by the back-propagation.  Cells that keep the level
+    from three buffers back (the grid's outer two columns and bottom two rows)
+    move one more level; they are left out (under 2% of the cells)."""
+    total = 0
+    for s in shots:
+        n = s.nx * s.nz
+        nb = -(-s.nt // T)
+        per_phase = 8 if nb == 1 else 16 + 20 + 28 * (nb - 2)
+        nsnap = (s.nt - 1) // k + 1
+        total += 2 * n * per_phase + 2 * n * 4 * nsnap
+    return total
+
+
+def counted_traffic(name):
+    """HBM bytes per cell-update of a kernel from a committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
+    p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        return json.load(f)
 
 
 def main():
@@ -534,7 +647,16 @@ def main():
     ap.add_argument("--served-slots", type=int, default=32)
     ap.add_argument("--cpu-budget", type=float, default=25.0,
                     help="G cell-updates of the CPU baseline sample (about 10 s)")
+    ap.add_argument("--cpu-budget-one-core", type=float, default=3.0,
+                    help="G cell-updates of the one-core CPU sample (the first shot)")
+    ap.add_argument("--no-check", action="store_true",
+                    help="skip comparing two shots of the timed batch with the per-step path")
+    ap.add_argument("--cpu-worker", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-worker-lite", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_worker:
+        cpu_worker(args.cpu_worker, args.cpu_budget * 1e9, not args.cpu_worker_lite)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU design "
               "(shots sharded across GPUs, one image reduce) is not part of this build",
@@ -568,12 +690,22 @@ def main():
     info = batch.info()
     log(f"batch resident in HBM in {time.time() - t0:.1f}s: {info}")
 
-    for _ in range(args.warmup):
+    # one step: migrate every shot, then stack the per-shot images and
+    # illuminations in FP32 into the extended grid at the window offsets
+    gnz, gnx = cfg.nz, cfg.global_nx
+    st_img = torch.empty((gnz, gnx), dtype=torch.float32, device="cuda")
+    st_ill = torch.empty_like(st_img)
+
+    def step():
         batch.run("batched")
+        batch.stack_device(st_img, st_ill)
+
+    for _ in range(args.warmup):
+        step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        batch.run("batched")
+        step()
         log(f"step {i} done")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -582,8 +714,21 @@ def main():
     shots_per_s = cfg.n_shots * args.steps / elapsed
 
     kernel_ms, launches = batch.time_kernels()
-    achieved = BYTES_PER_CELL_UPDATE * cu / (kernel_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles()
+    from das_amd.batch import kernel_configs
+
+    T = kernel_configs()[args.kernel][0] if args.kernel > 0 else 1
+    if args.kernel > 0:
+        alg = blocked_algorithmic_bytes(cfg.shots, T, cfg.k)
+    else:
+        alg = BYTES_PER_CELL_UPDATE * cu
+    achieved = alg / (kernel_ms * 1e-3) / 1e9
+    effective = BYTES_PER_CELL_UPDATE * cu / (kernel_ms * 1e-3) / 1e9
+    counted = counted_traffic("k_tb_hbm_traffic.json") if args.kernel > 0 else None
+    if counted is not None and counted.get("kernel_config") != args.kernel:
+        counted = None
+    # HBM bytes per launch: the blocked scheme's algorithmic bytes times the
+    # counted/algorithmic ratio of the committed rocprofv3 --pmc summary
+    traffic = round(counted["counted_over_algorithmic"] * alg / launches) if counted else None
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -601,11 +746,33 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "k_tb" if args.kernel else "k_batch_step",
+                     "kernel": (f"k_tb configuration {args.kernel} (T={T})" if args.kernel
+                                else "k_batch_step"),
                      "launches": launches,
                      "avg_launch_us": round(kernel_ms * 1e3 / max(launches, 1), 3),
-                     "algorithmic_bytes_per_cell_update": BYTES_PER_CELL_UPDATE},
+                     "algorithmic_bytes_per_launch": round(alg / launches),
+                     "algorithmic_bytes_per_cell_update": round(alg / cu, 3),
+                     "algorithmic_model": (f"blocked scheme: 28 B per cell per {T}-step block "
+                                           f"(16 read, 12 written) + 4 B per snapshot level "
+                                           f"each way (k={cfg.k})" if args.kernel else
+                                           "per-step scheme: 20 B per step + 24 B per imaging "
+                                           "step"),
+                     "effective_gbs": round(effective, 1),
+                     "effective_note": ("26 B per cell-update (the per-step scheme's bytes) "
+                                        "over the kernel time: the rate a per-step kernel "
+                                        "would need, not bytes moved")},
     }
+    if counted is not None:
+        gbs = traffic * launches / (kernel_ms * 1e-3) / 1e9
+        out["roofline"]["traffic_counted"] = {
+            "hbm_bytes_per_launch": traffic,
+            "bytes_per_cell_update": round(traffic * launches / cu, 3),
+            "counted_over_algorithmic": counted["counted_over_algorithmic"],
+            "gbs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+            "source": "profiles/k_tb_hbm_traffic.json (rocprofv3 FETCH_SIZE, WRITE_SIZE, "
+                      "copy-kernel calibrated)"}
+    if args.kernel > 0 and not args.no_check:
+        out["check"] = check_against_per_step(cfg, batch, (0, cfg.n_shots - 1), log)
     if not args.no_plain:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -616,7 +783,7 @@ def main():
                             "shots_per_s": round(cfg.n_shots / tp, 3),
                             "mode": "one shot at a time, one launch per time step"}
         out["speedup_vs_plain"] = round(shots_per_s / (cfg.n_shots / tp), 2)
-        out["plain_gpu"]["acoustic_shot_200x400_ms"] = plain_shot_200x400_ms()
+        out["plain_gpu"]["acoustic_shot_200x400"] = plain_shot_200x400()
     batch.close()
     if not args.no_elastic:
         out["elastic"] = elastic_leg(args, log)
@@ -628,7 +795,7 @@ def main():
         el = out.get("elastic", {}).get("value", 50.0)
         out["served"] = served_leg(args, value, el, args.kernel, log)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget * 1e9)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_budget * 1e9, args.cpu_budget_one_core * 1e9)
     print(json.dumps(out), flush=True)
 
 

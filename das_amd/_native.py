@@ -52,6 +52,7 @@ SIGNATURES = {
     "tsw_batch_time_kernels": (I, [P, I, ctypes.POINTER(D), ctypes.POINTER(I)]),
     "tsw_batch_shot_result": (I, [P, I, P, P]),
     "tsw_batch_stack": (I, [P, I, I, P, P]),
+    "tsw_batch_stack_device": (I, [P, I, I, P, P, P]),
     "tsw_batch_info": (I, [P, ctypes.POINTER(LL), ctypes.POINTER(LL), ctypes.POINTER(I),
                            ctypes.POINTER(I), ctypes.POINTER(LL)]),
     "tsw_batch_destroy": (I, [P]),

@@ -119,6 +119,23 @@ class ShotBatch:
         N.check(N.lib().tsw_batch_stack(self._h, int(gnx), int(gnz), N.ptr(img), N.ptr(ill)))
         return img, ill
 
+    def stack_device(self, image, illum, stream=None):
+        """Stacks into device tensors ``image`` and ``illum`` (float32, (gnz, gnx),
+        contiguous) on ``stream`` (torch's current stream by default)."""
+        import torch
+
+        for t in (image, illum):
+            if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() or t.dim() != 2:
+                raise ValueError("stack_device needs contiguous 2-D float32 device tensors")
+        if image.shape != illum.shape:
+            raise ValueError("image and illumination shapes differ")
+        gnz, gnx = image.shape
+        st = (stream or torch.cuda.current_stream()).cuda_stream
+        N.check(N.lib().tsw_batch_stack_device(self._h, int(gnx), int(gnz),
+                                               ctypes.c_void_p(image.data_ptr()),
+                                               ctypes.c_void_p(illum.data_ptr()),
+                                               ctypes.c_void_p(st)))
+
     def info(self) -> dict:
         cu, nb, ecu = N.LL(), N.LL(), N.LL()
         steps, graphs = N.I(), N.I()

@@ -817,6 +817,15 @@ int tsw_batch_stack(tsw_batch_t h, int gnx, int gnz, float* image_out, float* il
   return kOk;
 }
 
+int tsw_batch_stack_device(tsw_batch_t h, int gnx, int gnz, float* d_image, float* d_illum,
+                           void* stream) {
+  TSW_ARG(d_image && d_illum, "bad argument");
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return batch_stack_device(h, gnx, gnz, d_image, d_illum, st);
+}
+
 int tsw_batch_info(tsw_batch_t h, long long* cell_updates, long long* device_bytes,
                    int* total_steps, int* n_graphs, long long* elastic_cell_updates) {
   Batch* B = reinterpret_cast<Batch*>(h);

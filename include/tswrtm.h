@@ -212,6 +212,11 @@ int tsw_batch_shot_result(tsw_batch_t batch, int shot, float* image_out, float* 
 /* FP32 stack, in shot order, of every shot's image and illumination into a
  * (gnz, gnx) grid at the shots' window offsets. */
 int tsw_batch_stack(tsw_batch_t batch, int gnx, int gnz, float* image_out, float* illum_out);
+/* The same stack into caller-owned device arrays (gnz * gnx floats each),
+ * enqueued on `stream` (a hipStream_t; NULL: the library's stream) without
+ * waiting for it.  Replaces the stacking loop of rtm_engine.py:142-143. */
+int tsw_batch_stack_device(tsw_batch_t batch, int gnx, int gnz, float* d_image, float* d_illum,
+                           void* stream);
 /* Cell-updates per run, device bytes held, global steps, captured graphs
  * (both lanes), and the elastic lane's share of the cell-updates. */
 int tsw_batch_info(tsw_batch_t batch, long long* cell_updates, long long* device_bytes,
