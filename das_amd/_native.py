@@ -86,6 +86,9 @@ SIGNATURES = {
     "tsw_save_matrix_csv": (I, [ctypes.c_char_p, P, I, I, I, P, P, ctypes.c_char_p,
                                 ctypes.c_char_p]),
     "tsw_xcorr_device": (I, [P, I, I, P, I, P, I, I, I, P, P]),
+    "tsw_xcorr_device_noalloc": (I, [P, I, I, P, I, P, I, I, I, P, P]),
+    "tsw_xcorr_noalloc_scratch_bytes": (LL, [I, I, I, I]),
+    "tsw_xcorr_noalloc_scratch_max": (LL, []),
     "tsw_time_derivative": (I, [P, I, I, D, I, P]),
     "tsw_green_branch": (I, [P, I, I, I, I, P, ctypes.POINTER(I)]),
     "tsw_resample_linear": (I, [P, I, I, D, D, I, P]),

@@ -229,29 +229,59 @@ static unsigned grid1(long long n) {
   return (unsigned)std::max<long long>(1, std::min<long long>(b, 65536));
 }
 
-struct XcorrScratch {
-  DevBuf chs, S, R, pairs, S64;
+// Device areas of one correlation call: the channel list, the source and
+// receiver columns (ncol, nt), the pair-major lag sums and the FP64 source rows.
+struct XcorrAreas {
+  int* chs;
+  float* S;
+  float* R;
+  float* pairs;
+  double* S64;
 };
 
-static int xcorr_device(const float* d_data, int nt, int nch, const int* src_ch, int nsrc,
-                        const int* rec_ch, int nrec, int nlag, int accumulate, float* d_out,
-                        hipStream_t st) {
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of each area, in the order they are carved from one block.
+static void xcorr_area_bytes(int nt, int nsrc, int nrec, int nlag, size_t b[5]) {
+  const size_t nl = 2 * (size_t)nlag + 1;
+  b[0] = (size_t)(nsrc + nrec) * sizeof(int);
+  b[1] = (size_t)nsrc * nt * sizeof(float);
+  b[2] = (size_t)nrec * nt * sizeof(float);
+  b[3] = (size_t)nsrc * nrec * nl * sizeof(float);
+  b[4] = (size_t)nsrc * ((size_t)nt + kXC) * sizeof(double);
+}
+
+static size_t xcorr_scratch_bytes(int nt, int nsrc, int nrec, int nlag) {
+  size_t b[5], total = 0;
+  xcorr_area_bytes(nt, nsrc, nrec, nlag, b);
+  for (size_t v : b) total += align256(v);
+  return total;
+}
+
+static XcorrAreas xcorr_carve(char* base, int nt, int nsrc, int nrec, int nlag) {
+  size_t b[5];
+  xcorr_area_bytes(nt, nsrc, nrec, nlag, b);
+  char* q[5];
+  for (int i = 0; i < 5; ++i) {
+    q[i] = base;
+    base += align256(b[i]);
+  }
+  return XcorrAreas{(int*)q[0], (float*)q[1], (float*)q[2], (float*)q[3], (double*)q[4]};
+}
+
+// One correlation call on the areas `a`; returns after the stream has
+// drained, so the areas are free for the next call.
+static int xcorr_run(const float* d_data, int nt, int nch, const int* src_ch, int nsrc,
+                     const int* rec_ch, int nrec, int nlag, int accumulate, float* d_out,
+                     const XcorrAreas& a, hipStream_t st) {
   const int nl = 2 * nlag + 1;
-  // scratch kept across calls (grown as needed, held until the process ends);
-  // the entry points hold the API lock and drain the stream before returning
-  static XcorrScratch* scr = new XcorrScratch();
-  DevBuf &chs = scr->chs, &S = scr->S, &R = scr->R, &pairs = scr->pairs;
   std::vector<int> h(src_ch, src_ch + nsrc);
   h.insert(h.end(), rec_ch, rec_ch + nrec);
-  TSW_TRY(chs.ensure(h.size() * sizeof(int)));
-  TSW_TRY(S.ensure((size_t)nsrc * nt * sizeof(float)));
-  TSW_TRY(R.ensure((size_t)nrec * nt * sizeof(float)));
-  TSW_TRY(pairs.ensure((size_t)nsrc * nrec * nl * sizeof(float)));
-  TSW_HIP(hipMemcpyAsync(chs.p, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  TSW_HIP(hipMemcpyAsync(a.chs, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_take_cols, dim3((nt + 31) / 32, (nsrc + 31) / 32), dim3(32, 8), 0, st,
-                     d_data, nt, nch, chs.i(), nsrc, S.f());
+                     d_data, nt, nch, a.chs, nsrc, a.S);
   hipLaunchKernelGGL(k_take_cols, dim3((nt + 31) / 32, (nrec + 31) / 32), dim3(32, 8), 0, st,
-                     d_data, nt, nch, chs.i() + nsrc, nrec, R.f());
+                     d_data, nt, nch, a.chs + nsrc, nrec, a.R);
   // lags per lane: the smallest of 4 / 6 / 8 whose 64-lane tiles cover nl
   // with the least idle lanes
   int L = 8;
@@ -261,24 +291,59 @@ static int xcorr_device(const float* d_data, int nt, int nch, const int* src_ch,
   // the source rows in FP64, padded with zeros so that the last chunk (it
   // starts below nt and spans at most kXC samples) stays inside its row
   const int ntp = nt + kXC;
-  DevBuf& S64 = scr->S64;
-  TSW_TRY(S64.ensure((size_t)nsrc * ntp * sizeof(double)));
-  hipLaunchKernelGGL(k_rows_f64, dim3(grid1((long long)nsrc * ntp)), dim3(256), 0, st, S.f(), nt,
-                     ntp, nsrc, (double*)S64.p);
-  const double* s64 = (const double*)S64.p;
+  hipLaunchKernelGGL(k_rows_f64, dim3(grid1((long long)nsrc * ntp)), dim3(256), 0, st, a.S, nt,
+                     ntp, nsrc, a.S64);
+  const double* s64 = a.S64;
   if (L == 4)
-    hipLaunchKernelGGL(k_xcorr<4>, dim3(nrec, nsrc, tiles), dim3(64), 0, st, s64, ntp, R.f(), nt,
-                       nlag, nrec, pairs.f());
+    hipLaunchKernelGGL(k_xcorr<4>, dim3(nrec, nsrc, tiles), dim3(64), 0, st, s64, ntp, a.R, nt,
+                       nlag, nrec, a.pairs);
   else if (L == 6)
-    hipLaunchKernelGGL(k_xcorr<6>, dim3(nrec, nsrc, tiles), dim3(64), 0, st, s64, ntp, R.f(), nt,
-                       nlag, nrec, pairs.f());
+    hipLaunchKernelGGL(k_xcorr<6>, dim3(nrec, nsrc, tiles), dim3(64), 0, st, s64, ntp, a.R, nt,
+                       nlag, nrec, a.pairs);
   else
-    hipLaunchKernelGGL(k_xcorr<8>, dim3(nrec, nsrc, tiles), dim3(64), 0, st, s64, ntp, R.f(), nt,
-                       nlag, nrec, pairs.f());
+    hipLaunchKernelGGL(k_xcorr<8>, dim3(nrec, nsrc, tiles), dim3(64), 0, st, s64, ntp, a.R, nt,
+                       nlag, nrec, a.pairs);
   hipLaunchKernelGGL(k_lag_major, dim3((nl + 31) / 32, (nrec + 31) / 32, nsrc), dim3(32, 8), 0,
-                     st, pairs.f(), nrec, nl, d_out, accumulate);
+                     st, a.pairs, nrec, nl, d_out, accumulate);
   TSW_HIP(hipGetLastError());
-  TSW_HIP(hipStreamSynchronize(st));  // the scratch buffers are free for the next call
+  TSW_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+static int xcorr_device(const float* d_data, int nt, int nch, const int* src_ch, int nsrc,
+                        const int* rec_ch, int nrec, int nlag, int accumulate, float* d_out,
+                        hipStream_t st) {
+  // scratch kept across calls (grown as needed, held until the process ends);
+  // the entry points hold the API lock and drain the stream before returning
+  static DevBuf* scr = new DevBuf();
+  TSW_TRY(scr->ensure(xcorr_scratch_bytes(nt, nsrc, nrec, nlag)));
+  const XcorrAreas a = xcorr_carve((char*)scr->p, nt, nsrc, nrec, nlag);
+  return xcorr_run(d_data, nt, nch, src_ch, nsrc, rec_ch, nrec, nlag, accumulate, d_out, a, st);
+}
+
+// Fixed scratch of the allocation-free entry point: one array per device,
+// held from the library's load until the process ends.  Host code never uses
+// the name `scratch` as an address; it uses the device address that
+// hipGetSymbolAddress returns for the current device.
+constexpr size_t kXcorrScratchMax = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kXcorrScratchMax];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_dev[kMaxDevices];
+
+// Looks up (once per device, with that device current) the device address of
+// this device's `scratch`.
+int xcorr_scratch_setup(void** out) {
+  int dev = 0;
+  TSW_HIP(hipGetDevice(&dev));
+  TSW_ARG(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d the library tracks", dev,
+          kMaxDevices);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    TSW_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TSW_ARG(p != nullptr, "no device address for the correlation scratch");
+    g_scratch_dev[dev] = p;
+  }
+  if (out) *out = g_scratch_dev[dev];
   return kOk;
 }
 
@@ -341,6 +406,33 @@ int tsw_xcorr_device(const float* d_data, int nt, int nch, const int* src_ch, in
   return xcorr_device(d_data, nt, nch, src_ch, nsrc, rec_ch, nrec, nlag, accumulate, d_out,
                       static_cast<hipStream_t>(stream));
 }
+
+int tsw_xcorr_device_noalloc(const float* d_data, int nt, int nch, const int* src_ch, int nsrc,
+                             const int* rec_ch, int nrec, int nlag, int accumulate, float* d_out,
+                             void* stream) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  TSW_ARG(d_data && d_out, "null argument");
+  TSW_TRY(xcorr_args(nt, nch, src_ch, nsrc, rec_ch, nrec, nlag));
+  const size_t need = xcorr_scratch_bytes(nt, nsrc, nrec, nlag);
+  TSW_ARG(need <= kXcorrScratchMax,
+          "correlation needs %zu bytes of scratch, more than the %zu of the allocation-free "
+          "entry point (use tsw_xcorr_device, or fewer channels or samples per call)",
+          need, kXcorrScratchMax);
+  Ctx* ctx;
+  TSW_TRY(ctx_get(&ctx));
+  void* base = nullptr;
+  TSW_TRY(xcorr_scratch_setup(&base));
+  const XcorrAreas a = xcorr_carve((char*)base, nt, nsrc, nrec, nlag);
+  return xcorr_run(d_data, nt, nch, src_ch, nsrc, rec_ch, nrec, nlag, accumulate, d_out, a,
+                   static_cast<hipStream_t>(stream));
+}
+
+long long tsw_xcorr_noalloc_scratch_bytes(int nt, int nsrc, int nrec, int nlag) {
+  if (nt < 1 || nsrc < 1 || nrec < 1 || nlag < 0) return -1;
+  return (long long)xcorr_scratch_bytes(nt, nsrc, nrec, nlag);
+}
+
+long long tsw_xcorr_noalloc_scratch_max(void) { return (long long)kXcorrScratchMax; }
 
 int tsw_time_derivative(const float* in, int nl, int ncol, double dt, int scheme, float* out) {
   std::lock_guard<std::recursive_mutex> lk(api_mutex());

@@ -314,6 +314,24 @@ int tsw_xcorr(const float* data, int nt, int nch, const int* src_ch, int nsrc, c
 int tsw_xcorr_device(const float* d_data, int nt, int nch, const int* src_ch, int nsrc,
                      const int* rec_ch, int nrec, int nlag, int accumulate, float* d_out,
                      void* stream);
+/* Allocation-free variant of tsw_xcorr_device, with no counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results, bit
+ * for bit; the channel gathers, FP64 source rows and pair-major lag sums live
+ * in a fixed 64 MiB device array per GPU (held from the library's load until
+ * the process ends) instead of library-allocated buffers, and the call makes
+ * no hipMalloc/hipFree.  Returns kErrArg (-1) when the call would need more
+ * than that (tsw_xcorr_noalloc_scratch_bytes tells the need).  Every call on
+ * one GPU uses that GPU's array: calls that overlap on one GPU (from several
+ * host threads without the library's lock, or on several streams) race; the
+ * call returns after its stream has drained, so calls in sequence are safe. */
+int tsw_xcorr_device_noalloc(const float* d_data, int nt, int nch, const int* src_ch, int nsrc,
+                             const int* rec_ch, int nrec, int nlag, int accumulate, float* d_out,
+                             void* stream);
+/* Scratch bytes tsw_xcorr_device_noalloc needs for these sizes (-1 for bad
+ * sizes), and the size of its fixed array. */
+long long tsw_xcorr_noalloc_scratch_bytes(int nt, int nsrc, int nrec, int nlag);
+long long tsw_xcorr_noalloc_scratch_max(void);
 /* time_derivative_greens (interferometry.py:82-104) of (nl, ncol) float32:
  * scheme 0 "central" (np.gradient, edge order 1), 1 spectral (irfft of
  * i*omega*rfft over the next power of two). */
