@@ -14,7 +14,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libtswrtm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["acoustic.hip", "elastic.hip", "post.hip", "engine.hip", "batch.hip", "elabatch.hip", "serve.hip", "fwi.hip", "vgather.hip", "models.hip", "output.hip", "dsp.hip"]
+SOURCES = ["acoustic.hip", "elastic.hip", "post.hip", "engine.hip", "batch.hip", "elabatch.hip", "serve.hip", "fwi.hip", "vgather.hip", "models.hip", "output.hip", "dsp.hip", "ts_a.hip", "ts_b.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -20,6 +20,7 @@
 #include "elabatch.h"
 #include "post.h"
 #include "tb.h"
+#include "ts.h"
 #include "tswrtm.h"
 
 namespace tsw {
@@ -209,6 +210,10 @@ struct TbCfg {
 };
 #define TB_CFG(T, R, WX, WZ) \
   { T, R, WX, WZ, TbShape<T, R, WX, WZ>::W, TbShape<T, R, WX, WZ>::H, launch_tb<T, R, WX, WZ> }
+// z-streaming configurations (ts.h): WZ = 1, H rows per segment
+TSW_TS_CONFIGS_A(TSW_TS_EXTERN)
+TSW_TS_CONFIGS_B(TSW_TS_EXTERN)
+#define TS_CFG(T, R, WX, H) {T, R, WX, 1, TsShape<T, R, WX>::W, H, launch_ts<T, R, WX, H>},
 static const TbCfg kTbCfgs[] = {
     TB_CFG(4, 8, 2, 8),  // 1: 128 x 64 extended, 112 x 48 core, 1024 threads
     TB_CFG(2, 8, 2, 4),  // 2: 128 x 32, 120 x 24 core
@@ -226,6 +231,8 @@ static const TbCfg kTbCfgs[] = {
     TB_CFG(4, 6, 1, 16),  // 14: 64 x 96, 48 x 80 core
     TB_CFG(5, 4, 1, 16),  // 15: 64 x 64, 44 x 44 core
     TB_CFG(6, 4, 1, 16),  // 16: 64 x 64, 40 x 40 core
+    // 17..24: z-streaming strips of 64 WX columns, depth segments of H rows
+    TSW_TS_CONFIGS_A(TS_CFG) TSW_TS_CONFIGS_B(TS_CFG)
 };
 constexpr int kNumTbCfgs = (int)(sizeof(kTbCfgs) / sizeof(kTbCfgs[0]));
 
@@ -236,6 +243,10 @@ int tb_config(int id, int* T, int* W, int* H) {
   *W = c.W;
   *H = c.H;
   return kOk;
+}
+
+int tb_max_sponge(int id) {
+  return kTbCfgs[id - 1].WZ == 1 ? kTsMaxSponge : 1 << 30;  // k_ts: WZ 1
 }
 
 void tb_launch(int id, const BatchConst& bc, const TbShot* shots, const TbEntry* ents,
@@ -470,6 +481,8 @@ int tsw_batch_create(int nshot, const tsw_shot_desc* shots, double dx, double dz
   TSW_ARG(out && shots && nshot >= 1, "need at least one shot");
   TSW_ARG(k >= 1 && n_sponge >= 0 && dx > 0 && dz > 0 && dt > 0, "bad batch constants");
   TSW_ARG(kernel >= 0 && kernel <= kNumTbCfgs, "kernel %d outside [0, %d]", kernel, kNumTbCfgs);
+  TSW_ARG(kernel == 0 || n_sponge <= tb_max_sponge(kernel),
+          "kernel %d holds a sponge of at most %d cells", kernel, tb_max_sponge(kernel));
   *out = nullptr;
   std::vector<tsw_shot_desc> ac;
   std::vector<const tsw_shot_desc*> el;

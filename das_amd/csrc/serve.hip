@@ -493,6 +493,8 @@ int tsw_server_create(double dx, double dz, double dt, int n_sponge, double spon
   int T, W, H;
   TSW_ARG(tb_config(kernel, &T, &W, &H) == kOk, "kernel %d is not a blocked configuration",
           kernel);
+  TSW_ARG(n_sponge <= tb_max_sponge(kernel), "kernel %d holds a sponge of at most %d cells",
+          kernel, tb_max_sponge(kernel));
   Ctx* ctx;
   TSW_TRY(ctx_get(&ctx));
   auto S = std::make_unique<Server>();

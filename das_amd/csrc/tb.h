@@ -66,6 +66,8 @@ __device__ __forceinline__ int xcd_tile_tb(int b, int n) {
 // Configuration `id` (1-based) of the instantiated blocked kernels: steps per
 // launch and core tile; and a launch of it (defined in batch.hip).
 int tb_config(int id, int* T, int* W, int* H);
+// Widest sponge strip (cells) configuration `id` supports.
+int tb_max_sponge(int id);
 void tb_launch(int id, const BatchConst& bc, const TbShot* shots, const TbEntry* ents,
                const int* pre, int nent, int ntiles, hipStream_t st);
 

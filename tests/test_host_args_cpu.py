@@ -64,3 +64,36 @@ def test_engine_binds_torch_hip_runtime():
     mine = ctypes.cast(N.lib().hipGetDevice, ctypes.c_void_p).value
     theirs = ctypes.cast(ctypes.CDLL(hip).hipGetDevice, ctypes.c_void_p).value
     assert mine == theirs
+
+
+def test_streaming_kernels_refuse_wide_sponges():
+    """The z-streaming blocked configurations keep the sponge table in LDS
+    (at most 255 cells); a wider strip is refused with a bad-argument status
+    before anything touches a device, while the tile configurations accept
+    it."""
+    import ctypes
+
+    from das_amd import _native as N
+
+    L = N.lib()
+    ts = [i for i in range(1, L.tsw_kernel_configs() + 1)
+          if L.tsw_kernel_config(i, None, None, None, None) == 0 and _wz1(L, i)]
+    assert ts, "no streaming configuration in the table"
+    d = (N.ShotDesc * 1)()
+    for kid in ts:
+        h = N.P()
+        rc = L.tsw_batch_create(1, d, 5.0, 5.0, 5e-4, 256, 0.0015, 1, 2, 1, 0, kid,
+                                ctypes.byref(h))
+        assert rc == -1 and "sponge" in N.last_error(), (kid, N.last_error())
+
+
+def _wz1(L, i):
+    """Configuration i is a streaming one: one row of threads spans the strip
+    (core + 4T halo columns = threads), where a tile configuration has
+    several rows of threads."""
+    import ctypes
+
+    v = [ctypes.c_int() for _ in range(4)]
+    L.tsw_kernel_config(i, *[ctypes.byref(x) for x in v])
+    T, W, H, threads = (x.value for x in v)
+    return W + 4 * T == threads
