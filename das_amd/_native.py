@@ -123,6 +123,11 @@ class TswError(RuntimeError):
         self.status = status
 
 
+class WavefieldDiverged(TswError, FloatingPointError):
+    """Status -6: a finished shot's image or illumination holds NaN or Inf,
+    which is what an unstable time step (dt above the CFL limit) produces."""
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -167,6 +172,8 @@ def check(status):
         raise ValueError(msg)
     if status == -3:
         raise MemoryError(msg)
+    if status == -6:
+        raise WavefieldDiverged(status, msg)
     raise TswError(status, msg)
 
 

@@ -690,6 +690,10 @@ static int run_acoustic(Batch* B, int mode) {
   return kOk;
 }
 
+static const ShotHost& shot_host(const Batch* B, int i);
+static const float* shot_image(Batch* B, int i);
+static const float* shot_illum(Batch* B, int i);
+
 int tsw_batch_run(tsw_batch_t h, int mode) {
   Batch* B = reinterpret_cast<Batch*>(h);
   TSW_ARG(B, "null batch");
@@ -700,7 +704,17 @@ int tsw_batch_run(tsw_batch_t h, int mode) {
   if (B->nshot > 0) TSW_TRY(run_acoustic(B, mode));
   if (B->nshot > 0) TSW_HIP(hipStreamSynchronize(B->stream));
   if (B->ela) TSW_TRY(ela_lane_sync(B->ela));
-  return kOk;
+  // divergence sentinel: every shot's image and illumination must be finite
+  std::vector<const float*> arr;
+  std::vector<long long> len;
+  for (int i = 0; i < B->ntotal; ++i) {
+    arr.push_back(shot_image(B, i));
+    arr.push_back(shot_illum(B, i));
+    len.push_back(shot_host(B, i).cells());
+    len.push_back(shot_host(B, i).cells());
+  }
+  return check_finite(arr.data(), len.data(), (int)arr.size(), 2, 0, "shot",
+                      B->nshot > 0 ? B->stream : nullptr);
 }
 
 int tsw_batch_time_kernels(tsw_batch_t h, int lane, double* kernel_ms, int* launches) {

@@ -15,7 +15,8 @@ enum Status : int {
   kErrHip = -2,      // HIP runtime error
   kErrMem = -3,      // device allocation failed or budget exceeded
   kErrState = -4,    // call not valid in the object's current state
-  kErrInjected = -5  // failure injected on purpose (tsw_debug_fail_next)
+  kErrInjected = -5,  // failure injected on purpose (tsw_debug_fail_next)
+  kErrDiverged = -6   // non-finite image or illumination: the wavefield diverged
 };
 
 void set_error(const char* fmt, ...);

@@ -358,7 +358,9 @@ int tsw_migrate_shot_elastic(tsw_grid_t h, const float* const* model_fwd,
   TSW_HIP(hipMemcpyAsync(illum_out, g->illum.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
                          st));
   TSW_HIP(hipStreamSynchronize(st));
-  return kOk;
+  const float* arr[2] = {g->image.f(), g->illum.f()};
+  const long long len[2] = {n, n};
+  return check_finite(arr, len, 2, 2, 0, "elastic shot", st);
 }
 
 }  // extern "C"

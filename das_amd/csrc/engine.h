@@ -41,6 +41,7 @@ struct DevBuf {
   int ensure(size_t nbytes);
   float* f() const { return static_cast<float*>(p); }
   int* i() const { return static_cast<int*>(p); }
+  unsigned* u() const { return static_cast<unsigned*>(p); }
 };
 
 // Process-wide HIP state: the device and stream the engine runs on.
@@ -82,5 +83,12 @@ int grid_init(Grid* g, int nx, int nz, double dx, double dz, double dt, int n_sp
               double alpha, int fs);
 // Host table expf(-alpha*d*d), d = 0..n, as wave2d.c:258-262 evaluates it.
 void sponge_table(std::vector<float>& t, int n, float alpha);
+
+// Divergence sentinel: counts NaN / +-inf values in each of `narr` device
+// arrays (lengths n[i]) on stream `st` and waits for the counts.  The first
+// array with any returns kErrDiverged with a message naming `what` and the
+// index (index_base + i / per_item), otherwise kOk.
+int check_finite(const float* const* arrays, const long long* n, int narr, int per_item,
+                 int index_base, const char* what, hipStream_t st);
 
 }  // namespace tsw

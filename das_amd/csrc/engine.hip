@@ -89,6 +89,33 @@ int RecvMap::build(const int* rix, int nrec, int nx) {
   return kOk;
 }
 
+int check_finite(const float* const* arrays, const long long* n, int narr, int per_item,
+                 int index_base, const char* what, hipStream_t st) {
+  // counters kept across calls; the callers hold the library lock or the
+  // serving loop's worker thread, and each call drains its stream
+  static std::mutex mu;
+  static DevBuf* counts = new DevBuf();
+  std::lock_guard<std::mutex> lk(mu);
+  if (narr <= 0) return kOk;
+  TSW_TRY(counts->ensure((size_t)narr * sizeof(unsigned)));
+  launch_fill(counts->f(), narr, 0.0f, st);  // all-zero bits
+  for (int i = 0; i < narr; ++i)
+    launch_count_nonfinite(arrays[i], n[i], counts->u() + i, st);
+  std::vector<unsigned> h(narr);
+  TSW_HIP(hipGetLastError());
+  TSW_HIP(hipMemcpyAsync(h.data(), counts->p, narr * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  TSW_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < narr; ++i) {
+    if (h[i]) {
+      set_error("%s %d: %u non-finite image / illumination values: the wavefield diverged "
+                "(dt may exceed the CFL limit; see WaveSolver2D.check_stability)",
+                what, index_base + i / per_item, h[i]);
+      return kErrDiverged;
+    }
+  }
+  return kOk;
+}
+
 void sponge_table(std::vector<float>& t, int n, float alpha) {
   t.assign(n + 1, 1.0f);
   for (int d = 1; d <= n; ++d) {
@@ -506,7 +533,12 @@ int tsw_migrate_shot(tsw_grid_t h, const float* vel_fwd, const float* vel_bwd, c
     TSW_TRY(tsw_batch_create(1, &d, g->dx, g->dz, g->dt, g->n_sponge, g->alpha, g->fs, k, 1, 0,
                              kDefaultBlockedKernel, &b));
     int rc = tsw_batch_run(b, TSW_RUN_BATCHED);
-    if (rc == kOk) rc = tsw_batch_shot_result(b, 0, image_out, illum_out);
+    if (rc == kOk || rc == kErrDiverged) {  // a diverged shot's result is still copied out
+      const std::string err = rc ? std::string(tsw_last_error()) : std::string();
+      const int r2 = tsw_batch_shot_result(b, 0, image_out, illum_out);
+      if (r2 != kOk) rc = r2;
+      else if (rc) set_error("%s", err.c_str());
+    }
     tsw_batch_destroy(b);
     return rc;
   }
@@ -528,7 +560,9 @@ int tsw_migrate_shot(tsw_grid_t h, const float* vel_fwd, const float* vel_bwd, c
   TSW_HIP(hipMemcpyAsync(illum_out, g->illum.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
                          st));
   TSW_HIP(hipStreamSynchronize(st));
-  return kOk;
+  const float* arr[2] = {g->image.f(), g->illum.f()};
+  const long long len[2] = {n, n};
+  return check_finite(arr, len, 2, 2, 0, "shot", st);
 }
 
 int tsw_migrate_shots(tsw_grid_t h, const float* vel_fwd, const float* vel_bwd, int nshot,
@@ -569,6 +603,7 @@ int tsw_migrate_shots(tsw_grid_t h, const float* vel_fwd, const float* vel_bwd, 
   TSW_TRY(shot_ill.ensure(nb));
   TSW_TRY(out.ensure(nb));
   TSW_TRY(maxbits.ensure(sizeof(unsigned)));
+  std::string diverged;
   if (!g->custom_sponge && nshot >= 1) {
     // All shots through the batch engine (temporally blocked steps, every
     // shot resident at once as memory allows): each shot's image and
@@ -602,6 +637,12 @@ int tsw_migrate_shots(tsw_grid_t h, const float* vel_fwd, const float* vel_bwd, 
     TSW_TRY(tsw_batch_create(nshot, d.data(), g->dx, g->dz, g->dt, g->n_sponge, g->alpha, g->fs, k,
                              slots, 100, kDefaultBlockedKernel, &b));
     int rc = tsw_batch_run(b, TSW_RUN_BATCHED);
+    // a diverged shot still stacks (its NaN/Inf reach the outputs); the
+    // batch's message naming that shot is reported at the end
+    if (rc == kErrDiverged) {
+      diverged = tsw_last_error();
+      rc = kOk;
+    }
     if (rc == kOk) rc = batch_stack_device(b, g->nx, g->nz, stacked.f(), total.f(), st);
     if (rc == kOk && hipStreamSynchronize(st) != hipSuccess) rc = kErrHip;
     tsw_batch_destroy(b);
@@ -627,7 +668,13 @@ int tsw_migrate_shots(tsw_grid_t h, const float* vel_fwd, const float* vel_bwd, 
   TSW_HIP(hipMemcpyAsync(lap_out, shot_img.p, nb, hipMemcpyDeviceToHost, st));
   TSW_HIP(hipMemcpyAsync(norm_out, out.p, nb, hipMemcpyDeviceToHost, st));
   TSW_HIP(hipStreamSynchronize(st));
-  return kOk;
+  if (!diverged.empty()) {
+    set_error("%s", diverged.c_str());
+    return kErrDiverged;
+  }
+  const float* arr[2] = {stacked.f(), total.f()};
+  const long long len[2] = {n, n};
+  return check_finite(arr, len, 2, 2, 0, "stack of shots", st);
 }
 
 int tsw_laplacian(const float* image, float* out, int nx, int nz, double dx, double dz) {

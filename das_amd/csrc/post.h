@@ -25,6 +25,8 @@ void launch_stack_window(float* acc, int nx, const float* x, int wnx, int wnz, i
                          hipStream_t st);
 // Zeroes a (and b if non-null) where sponge < 0.99 (rtm_engine.py:147-149).
 void launch_mute(float* a, float* b, const float* sponge, long long n, hipStream_t st);
+// *count (zeroed by the caller) += number of NaN / +-inf values of x.
+void launch_count_nonfinite(const float* x, long long n, unsigned* count, hipStream_t st);
 // *out (zeroed by the caller) := bit pattern of max(x, 0).
 void launch_max_nonneg(const float* x, long long n, unsigned* out, hipStream_t st);
 void launch_normalize(const float* stacked, const float* illum, float* out, long long n,

@@ -55,6 +55,16 @@ __global__ void k_max_nonneg(const float* x, long long n, unsigned* out) {
   if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
+// *count += number of values that are NaN or +-inf (exponent bits all set).
+__global__ void k_count_nonfinite(const float* x, long long n, unsigned* count) {
+  unsigned c = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    c += (__float_as_uint(x[i]) & 0x7f800000u) == 0x7f800000u ? 1u : 0u;
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
 // normalized = stacked / (sqrt(illum) + eps), eps = 1e-4*max if max > 0 else 1e-12
 // (numpy scalar rules: every operand rounded to FP32).
 __global__ void k_normalize(const float* stacked, const float* illum, float* out, long long n,
@@ -94,6 +104,10 @@ void launch_stack_window(float* acc, int nx, const float* x, int wnx, int wnz, i
 
 void launch_mute(float* a, float* b, const float* sponge, long long n, hipStream_t st) {
   hipLaunchKernelGGL(k_mute, dim3(blocks_for(n)), dim3(256), 0, st, a, b, sponge, n);
+}
+
+void launch_count_nonfinite(const float* x, long long n, unsigned* count, hipStream_t st) {
+  hipLaunchKernelGGL(k_count_nonfinite, dim3(blocks_for(n)), dim3(256), 0, st, x, n, count);
 }
 
 void launch_max_nonneg(const float* x, long long n, unsigned* out, hipStream_t st) {

@@ -368,6 +368,15 @@ static int tick(Server* S, std::vector<Req*>& done, long long& cu, double& busy)
   }
   for (Req* r : act_e)
     if (S->g_e - r->e_start >= 2 * r->nt) done.push_back(r);
+  // divergence sentinel on the finished requests' images and illuminations
+  for (Req* r : done) {
+    const float* arr[2] = {r->image, r->illum};
+    const long long len[2] = {r->cells(), r->cells()};
+    if (check_finite(arr, len, 2, 2, (int)r->id, "request", S->sa) == kErrDiverged)
+      r->err = tsw_last_error();
+    else
+      r->err.clear();
+  }
   return kOk;
 }
 
@@ -460,6 +469,11 @@ static void worker_loop(Server* S) {
     S->busy_s += busy;
     const double t = S->now();
     for (Req* r : done) {
+      if (!r->err.empty()) {  // diverged: failed with the sentinel's reason
+        const std::string why = r->err;
+        fail_request(S, r, kFailed, why.c_str());
+        continue;
+      }
       r->state = kDone;
       r->t_done = t;
       S->completed++;

@@ -42,6 +42,16 @@ def apply_laplacian_filter(image: np.ndarray, dx: float, dz: float) -> np.ndarra
     return out
 
 
+def _check_kept(status, *arrays):
+    """N.check, except that a diverged shot's finished arrays (NaN/Inf in
+    them) ride on the exception as ``result`` for the caller to inspect."""
+    try:
+        N.check(status)
+    except N.WavefieldDiverged as e:
+        e.result = arrays
+        raise
+
+
 class TSWRTMEngine:
     """Transmitted-surface-wave RTM engine (tsw_rtm/rtm_engine.py:41-171)."""
 
@@ -93,9 +103,10 @@ class TSWRTMEngine:
         img = np.empty((s.nz, s.nx), np.float32)
         ill = np.empty((s.nz, s.nx), np.float32)
         s._sync_sponge()
-        N.check(N.lib().tsw_migrate_shot(s._grid, N.ptr(vf), N.ptr(vb), N.ptr(src), len(src),
-                                         six, siz, N.ptr(data), data.shape[0], N.ptr(rix),
-                                         len(rix), riz, k, N.ptr(img), N.ptr(ill)))
+        _check_kept(N.lib().tsw_migrate_shot(s._grid, N.ptr(vf), N.ptr(vb), N.ptr(src),
+                                             len(src), six, siz, N.ptr(data), data.shape[0],
+                                             N.ptr(rix), len(rix), riz, k, N.ptr(img),
+                                             N.ptr(ill)), img, ill)
         return img, ill
 
     def migrate_shot_elastic(self, model_fwd, model_bwd, source_ts, source_ix, source_iz,
@@ -114,9 +125,10 @@ class TSWRTMEngine:
         img = np.empty((s.nz, s.nx), np.float32)
         ill = np.empty_like(img)
         s._sync_sponge()
-        N.check(N.lib().tsw_migrate_shot_elastic(s._grid, PF, PB, N.ptr(src), six, siz,
-                                                 N.ptr(data), data.shape[0], N.ptr(rix), len(rix),
-                                                 riz, k, N.ptr(img), N.ptr(ill)))
+        _check_kept(N.lib().tsw_migrate_shot_elastic(s._grid, PF, PB, N.ptr(src), six, siz,
+                                                     N.ptr(data), data.shape[0], N.ptr(rix),
+                                                     len(rix), riz, k, N.ptr(img), N.ptr(ill)),
+                    img, ill)
         return img, ill
 
     def migrate_multiple_shots(self, flood_models: FloodModels, source_waveforms: List[np.ndarray],
@@ -148,10 +160,11 @@ class TSWRTMEngine:
         lap = np.empty_like(raw)
         norm = np.empty_like(raw)
         s._sync_sponge()
-        N.check(N.lib().tsw_migrate_shots(s._grid, N.ptr(vf), N.ptr(vb), n_shots, srcs, nt_src,
-                                          sixs, sizs, datas, nts, N.ptr(rix), len(rix), riz, k,
-                                          1 if mute_sponge else 0, N.ptr(raw), N.ptr(lap),
-                                          N.ptr(norm)))
+        _check_kept(N.lib().tsw_migrate_shots(s._grid, N.ptr(vf), N.ptr(vb), n_shots, srcs,
+                                              nt_src, sixs, sizs, datas, nts, N.ptr(rix),
+                                              len(rix), riz, k, 1 if mute_sponge else 0,
+                                              N.ptr(raw), N.ptr(lap), N.ptr(norm)),
+                    raw, lap, norm)
         return RTMResult(raw_image=raw, laplacian_image=lap, normalized_image=norm,
                          grid_x=np.arange(s.nx) * s.dx, grid_z=np.arange(s.nz) * s.dz,
                          dx=s.dx, dz=s.dz)

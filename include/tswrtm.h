@@ -15,7 +15,8 @@
  *   2. Coarse entry points that keep the time loop, snapshots and images in
  *      HBM.  They return 0 on success and a negative status on failure
  *      (-1 bad argument, -2 HIP error, -3 out of device memory / budget,
- *      -4 wrong state, -5 injected failure); tsw_last_error() describes it.
+ *      -4 wrong state, -5 injected failure, -6 diverged: a finished image or
+ *      illumination holds NaN/Inf); tsw_last_error() describes it.
  *      Host pointers are borrowed for the duration of the call.
  */
 #ifndef TSWRTM_H
