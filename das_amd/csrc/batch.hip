@@ -496,6 +496,9 @@ int tsw_batch_create(int nshot, const tsw_shot_desc* shots, double dx, double dz
             "shot %d: source outside grid", i);
     TSW_ARG(d.riz >= 0 && d.riz < d.nz && d.nrec >= 0, "shot %d: bad receiver row", i);
     TSW_ARG(d.nrec == 0 || (d.rix && d.data), "shot %d: receivers need rix and data", i);
+    for (int r = 0; r < d.nrec; ++r)
+      TSW_ARG(d.rix[r] >= 0 && d.rix[r] < d.nx, "shot %d: receiver %d column %d outside [0, %d)",
+              i, r, d.rix[r], d.nx);
     TSW_ARG((long long)d.nx * d.nz < (1LL << 31), "shot %d: grid too large", i);
     if (d.kind == TSW_SHOT_ACOUSTIC) {
       TSW_ARG(d.vel_fwd && d.vel_bwd, "shot %d: missing velocity models", i);

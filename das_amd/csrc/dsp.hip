@@ -431,13 +431,13 @@ extern "C" {
 int tsw_sosfiltfilt(const float* x, int nt, int nch, const double* sos, int nsec,
                     const double* zi, int padlen, int step, int sign, float* out) {
   std::lock_guard<std::recursive_mutex> lk(api_mutex());
-  TSW_ARG(x && out, "null argument");
+  TSW_ARG(x && out && zi, "null argument");
+  TSW_TRY(sos_check(nt, nch, sos, nsec, padlen, step));
   Ctx* ctx;
   TSW_TRY(ctx_get(&ctx));
   hipStream_t st = ctx->stream;
   const int nout = (nt + step - 1) / step;
   DevBuf &dx = scratch(0), &dout = scratch(2);
-  TSW_TRY(sos_check(nt, nch, sos, nsec, padlen, step));
   TSW_TRY(dx.ensure((size_t)nt * nch * sizeof(float)));
   TSW_TRY(dout.ensure((size_t)nout * nch * sizeof(float)));
   TSW_HIP(hipMemcpyAsync(dx.p, x, (size_t)nt * nch * sizeof(float), hipMemcpyHostToDevice, st));

@@ -389,6 +389,7 @@ int tsw_grid_create(int nx, int nz, double dx, double dz, double dt, int n_spong
 }
 
 int tsw_grid_destroy(tsw_grid_t h) {
+  if (!h) return kOk;
   std::lock_guard<std::recursive_mutex> lk(g_api_mu);
   Ctx* ctx;
   TSW_TRY(ctx_get(&ctx));

@@ -1316,6 +1316,17 @@ int tsw_fwi_create(int nx, int nz, double dx, double dz, double dt, int n_sponge
   TSW_ARG(out && shots && nshot >= 1, "need at least one shot");
   TSW_ARG(precision == 4 || precision == 8, "precision must be 4 (FP32) or 8 (FP64)");
   TSW_ARG(nx >= 5 && nz >= 5 && dx > 0 && dz > 0 && dt > 0 && n_sponge >= 0, "bad grid");
+  for (int i = 0; i < nshot; ++i) {
+    const tsw_fwi_shot& d = shots[i];
+    TSW_ARG(d.nt >= 1 && d.src && d.d_obs && d.rix && d.nrec >= 1,
+            "shot %d: need nt >= 1, nrec >= 1 and non-null arrays", i);
+    TSW_ARG(d.six >= 0 && d.six < nx && d.siz >= 0 && d.siz < nz,
+            "shot %d: source (%d, %d) outside the %dx%d grid", i, d.siz, d.six, nz, nx);
+    TSW_ARG(d.riz >= 0 && d.riz < nz, "shot %d: receiver row %d outside the grid", i, d.riz);
+    for (int r = 0; r < d.nrec; ++r)
+      TSW_ARG(d.rix[r] >= 0 && d.rix[r] < nx, "shot %d: receiver %d column %d outside [0, %d)",
+              i, r, d.rix[r], nx);
+  }
   *out = nullptr;
   Ctx* ctx;
   TSW_TRY(ctx_get(&ctx));
@@ -1336,17 +1347,7 @@ int tsw_fwi_create(int nx, int nz, double dx, double dz, double dt, int n_sponge
   const long long n = F->cells();
   const size_t es = F->esz();
   int ntmax = 1;
-  for (int i = 0; i < nshot; ++i) {
-    const tsw_fwi_shot& d = shots[i];
-    if (!(d.nt >= 1 && d.src && d.d_obs && d.rix && d.nrec >= 1))
-      return fail((set_error("shot %d: need nt >= 1, nrec >= 1 and non-null arrays", i), kErrArg));
-    if (!(d.six >= 0 && d.six < nx && d.siz >= 0 && d.siz < nz))
-      return fail((set_error("shot %d: source (%d, %d) outside the %dx%d grid", i, d.siz, d.six,
-                             nz, nx), kErrArg));
-    if (!(d.riz >= 0 && d.riz < nz))
-      return fail((set_error("shot %d: receiver row %d outside the grid", i, d.riz), kErrArg));
-    ntmax = std::max(ntmax, d.nt);
-  }
+  for (int i = 0; i < nshot; ++i) ntmax = std::max(ntmax, shots[i].nt);
   const bool store_all = ckpt_interval < 0;
   F->kint = ckpt_interval > 0 ? ckpt_interval
                               : std::max(1, (int)std::ceil(std::sqrt((double)ntmax)));
@@ -1528,6 +1529,7 @@ int tsw_fwi_probe(tsw_fwi_t h, int shot, void* host_out) {
 }
 
 int tsw_fwi_destroy(tsw_fwi_t h) {
+  if (!h) return kOk;
   std::lock_guard<std::recursive_mutex> lk(api_mutex());
   (void)hipDeviceSynchronize();
   delete reinterpret_cast<Fwi*>(h);

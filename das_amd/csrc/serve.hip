@@ -399,6 +399,7 @@ static void fail_request(Server* S, Req* r, int state, const char* err) {
   S->mem_used -= r->heavy_bytes + r->keep;
   release(S, r);
   release_keep(S, r);
+  S->cv.notify_all();  // waiters on this request (no tick may follow to wake them)
 }
 
 static void worker_loop(Server* S) {
@@ -700,10 +701,11 @@ int tsw_server_wait(tsw_server_t h, long long id, double timeout_s) {
       set_error("server stopped: %s", S->fatal.c_str());
       return kErrState;
     }
-    if (S->cv.wait_until(lk, until) == std::cv_status::timeout) {
+    if (std::chrono::steady_clock::now() >= until) {
       set_error("request %lld not finished after %.1f s", id, timeout_s);
       return kErrState;
     }
+    S->cv.wait_until(lk, until);  // the state is checked again on every wake
   }
 }
 
@@ -782,7 +784,7 @@ int tsw_server_stats(tsw_server_t h, long long* completed, long long* failed,
 
 int tsw_server_destroy(tsw_server_t h) {
   Server* S = reinterpret_cast<Server*>(h);
-  TSW_ARG(S, "null server");
+  if (!S) return kOk;
   {
     std::lock_guard<std::mutex> lk(S->mu);
     S->stop = true;

@@ -17,7 +17,8 @@
  *      (-1 bad argument, -2 HIP error, -3 out of device memory / budget,
  *      -4 wrong state, -5 injected failure, -6 diverged: a finished image or
  *      illumination holds NaN/Inf); tsw_last_error() describes it.
- *      Host pointers are borrowed for the duration of the call.
+ *      Host pointers are borrowed for the duration of the call.  The
+ *      *_destroy calls accept a null handle and then do nothing (return 0).
  */
 #ifndef TSWRTM_H
 #define TSWRTM_H

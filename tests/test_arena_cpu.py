@@ -38,10 +38,16 @@ def _sizes():
     return out, int(2.2 * max(need))
 
 
+SANITIZE = ["-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+            "-fno-sanitize-recover=all"]
+
+
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
-def test_arena_never_stalls_and_coalesces(tmp_path):
+@pytest.mark.parametrize("flags", [[], SANITIZE], ids=["plain", "asan_ubsan"])
+def test_arena_never_stalls_and_coalesces(tmp_path, flags):
     exe = tmp_path / "arena_check"
-    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "das_amd", "csrc"),
+    subprocess.run(["g++", "-std=c++17", "-O1", *flags, "-I",
+                    os.path.join(ROOT, "das_amd", "csrc"),
                     os.path.join(HERE, "cpp", "arena_check.cpp"), "-o", str(exe)], check=True)
     sizes, budget = _sizes()
     args = [str(v) for hk in sizes for v in hk] + [str(budget)]

@@ -153,6 +153,19 @@ def test_budget_rejection_cancel_and_injected_failure(reqs):
     srv.close()
 
 
+def test_lone_failed_admission_wakes_its_waiter(reqs):
+    """A request whose admission fails while nothing else runs (so no tick
+    follows) is reported failed at once, not when the wait times out."""
+    shots, _ = reqs
+    srv = _server(max_active=1)
+    N.lib().tsw_debug_fail_next(1)
+    rid = srv.submit(shots[2])
+    t0 = time.monotonic()
+    assert srv.wait(rid, timeout=60.0)["state"] == "failed"
+    assert time.monotonic() - t0 < 10.0
+    srv.close()
+
+
 def test_resident_model_windows_equal_uploaded_models():
     """Requests that name a window of the server's resident models give the
     results of the same requests carrying copies of those windows."""
