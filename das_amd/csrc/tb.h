@@ -124,43 +124,58 @@ __attribute__((amdgpu_waves_per_eu(kTbWavesPerEu<R, WX, WZ>))) void k_tb(
   const bool col_in = col < nx;
   const bool core_col = col_in && col >= cx0 && col < cx0 + S::W;
   const bool col_int = col >= 2 && col < nx - 2;
-  // per-row classification bit masks
+  // per-row classification bit masks (the row part is uniform over a wave)
   unsigned m_in = 0, m_int = 0, m_core = 0;
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int row = row0 + q;
-    if (col_in && row < nz) {
-      m_in |= 1u << q;
-      if (col_int && row >= 2 && row < nz - 2) m_int |= 1u << q;
-      if (core_col && row >= cz0 && row < cz0 + S::H) m_core |= 1u << q;
-    }
+    const bool rin = row < nz;
+    m_in |= (unsigned)(col_in && rin) << q;
+    m_int |= (unsigned)(col_int && row >= 2 && row < nz - 2) << q;
+    m_core |= (unsigned)(core_col && rin && row >= cz0 && row < cz0 + S::H) << q;
   }
   // rows 0/1 under a free surface are rewritten every step, never "kept"
   const unsigned m_fs = (bc.fs && row0 == 0) ? (m_in & 3u) : 0u;
   const unsigned m_keep = m_in & ~m_int & ~m_fs;
 
   const gfloat* coef = as_global(fwd ? P.coef_f : P.coef_b);
+  const gfloat* tab = as_global(bc.table);
   float a[R], bl[R], c[R], cf[R], sp[R], acc[R];
 
-  // ---- load (or generate) the three input levels
+  // ---- load (or generate) the three input levels.  Lanes past the grid read
+  // the clamped last column / row and discard it, so that the loads of all
+  // rows are issued together without branches; 32-bit element offsets (the
+  // batch admits grids of fewer than 2^31 cells).
+  const int colc = col_in ? col : nx - 1;
   const int rd = E.rd;
+  const gfloat* lv2 = as_global(P.set[rd][2]);
+  const gfloat* lv1 = as_global(P.set[rd][1]);
+  const gfloat* lv0 = as_global(P.set[rd][0]);
+  const gfloat* accp = as_global(fwd ? P.illum : P.image);
+  unsigned idx[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int row = row0 + q;
+    const int rowc = row < nz ? row : nz - 1;
+    idx[q] = (unsigned)(rowc * nx + colc);
+    const bool in = m_in >> q & 1u;
+    const float cfv = coef[idx[q]];
+    // depth of a clamped cell is at most n_sponge; table[0] = 1
+    const float spv = tab[sponge_depth(colc, rowc, nx, nz, bc.n_sponge, bc.fs)];
+    cf[q] = in ? cfv : 0.0f;
+    sp[q] = in ? spv : 1.0f;
     a[q] = 0.0f;
     bl[q] = 0.0f;
     c[q] = 0.0f;
-    cf[q] = 0.0f;
-    sp[q] = 1.0f;
     acc[q] = 0.0f;
-    if (m_in >> q & 1u) {
-      const int idx = row * nx + col;
-      cf[q] = coef[idx];
-      const int d = sponge_depth(col, row, nx, nz, bc.n_sponge, bc.fs);
-      if (d > 0) sp[q] = bc.table[d];
-      if (first) {
+  }
+  if (first) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      if (m_in >> q & 1u) {
+        const int row = row0 + q;
         if (fwd) {
-          if (idx == P.src_cell) c[q] = 0.0f + as_global(P.src)[0] * bc.inj;
+          if ((int)idx[q] == P.src_cell) c[q] = 0.0f + as_global(P.src)[0] * bc.inj;
         } else if (row == P.riz) {
           const int r0 = as_global(P.roff)[col], r1 = as_global(P.roff)[col + 1];
           const gfloat* drow = as_global(P.data) + (long long)(P.nt - 1) * P.nrec;
@@ -168,12 +183,20 @@ __attribute__((amdgpu_waves_per_eu(kTbWavesPerEu<R, WX, WZ>))) void k_tb(
           for (int r = r0; r < r1; ++r) v = v + drow[as_global(P.ridx)[r]] * bc.inj;
           c[q] = v;
         }
-      } else {
-        c[q] = as_global(P.set[rd][2])[idx];
-        bl[q] = as_global(P.set[rd][1])[idx];
-        if (m_keep >> q & 1u) a[q] = as_global(P.set[rd][0])[idx];
-        if (m_core >> q & 1u) acc[q] = as_global(fwd ? P.illum : P.image)[idx];
       }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const bool in = m_in >> q & 1u;
+      const float cv = lv2[idx[q]], bv = lv1[idx[q]];
+      c[q] = in ? cv : 0.0f;
+      bl[q] = in ? bv : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      if (m_keep >> q & 1u) a[q] = lv0[idx[q]];
+      if (m_core >> q & 1u) acc[q] = accp[idx[q]];
     }
   }
   float* X = lds[0];
@@ -201,7 +224,12 @@ __attribute__((amdgpu_waves_per_eu(kTbWavesPerEu<R, WX, WZ>))) void k_tb(
   int ph = E.it0 % k, lvl = E.it0 / k;
 
   const float w = bc.w;
-  for (int j = 0; j < E.nsub; ++j) {
+  // The sub-steps are unrolled (at most T of them), so the rotation of the
+  // three levels is a renaming of registers and each LDS buffer's address is
+  // fixed per sub-step.
+#pragma unroll
+  for (int j = 0; j < T; ++j) {
+    if (j == E.nsub) break;
     const int it = fwd ? E.it0 + j : E.it0 - j;
     float* Y = lds[(j + 1) & 1];
     // side work on the post-injection level `c`
@@ -212,8 +240,7 @@ __attribute__((amdgpu_waves_per_eu(kTbWavesPerEu<R, WX, WZ>))) void k_tb(
         for (int q = 0; q < R; ++q) {
           if (m_core >> q & 1u) {
             // snapshots stream through HBM once each way: keep them out of the caches
-            const unsigned off = (unsigned)((row0 + q) * nx + col);
-            __builtin_nontemporal_store(c[q], snap + off);
+            __builtin_nontemporal_store(c[q], snap + idx[q]);
             acc[q] = acc[q] + c[q] * c[q];
           }
         }
@@ -222,10 +249,8 @@ __attribute__((amdgpu_waves_per_eu(kTbWavesPerEu<R, WX, WZ>))) void k_tb(
         // one memory latency instead of queueing one after another
         float sv[R];
 #pragma unroll
-        for (int q = 0; q < R; ++q) {
-          const unsigned off = (unsigned)((row0 + q) * nx + col);
-          sv[q] = (m_core >> q & 1u) ? __builtin_nontemporal_load(snap + off) : 0.0f;
-        }
+        for (int q = 0; q < R; ++q)
+          sv[q] = (m_core >> q & 1u) ? __builtin_nontemporal_load(snap + idx[q]) : 0.0f;
 #pragma unroll
         for (int q = 0; q < R; ++q)
           if (m_core >> q & 1u) acc[q] = acc[q] + (sv[q] * c[q]) * w;
@@ -306,17 +331,19 @@ __attribute__((amdgpu_waves_per_eu(kTbWavesPerEu<R, WX, WZ>))) void k_tb(
 
   // ---- write back the core
   const int wr = rd ^ 1;
+  gfloat* acco = as_global(fwd ? P.illum : P.image);
+  gfloat* wv2 = as_global(P.set[wr][2]);
+  gfloat* wv1 = as_global(P.set[wr][1]);
+  gfloat* wv0 = as_global(P.set[wr][0]);
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     if (m_core >> q & 1u) {
-      const int idx = (row0 + q) * nx + col;
       if (!last) {
-        as_global(P.set[wr][2])[idx] = c[q];
-        as_global(P.set[wr][1])[idx] = bl[q];
-        if (m_keep >> q & 1u) as_global(P.set[wr][0])[idx] = a[q];
+        wv2[idx[q]] = c[q];
+        wv1[idx[q]] = bl[q];
+        if (m_keep >> q & 1u) wv0[idx[q]] = a[q];
       }
-      if (fwd) as_global(P.illum)[idx] = last ? acc[q] * w : acc[q];
-      else as_global(P.image)[idx] = acc[q];
+      acco[idx[q]] = (fwd && last) ? acc[q] * w : acc[q];
     }
   }
 }
