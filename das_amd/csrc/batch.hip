@@ -205,15 +205,16 @@ static void launch_tb(const BatchConst& bc, const TbShot* shots, const TbEntry* 
                      ents, pre, nent, ntiles);
 }
 struct TbCfg {
-  int T, R, WX, WZ, W, H;
+  int T, R, WX, WZ, W, H;  // k_ts: WZ = T + 1 wave rows (one per sub-step and the writer)
   TbLaunchFn fn;
+  bool streaming;
 };
 #define TB_CFG(T, R, WX, WZ) \
-  { T, R, WX, WZ, TbShape<T, R, WX, WZ>::W, TbShape<T, R, WX, WZ>::H, launch_tb<T, R, WX, WZ> }
-// z-streaming configurations (ts.h): WZ = 1, H rows per segment
+  { T, R, WX, WZ, TbShape<T, R, WX, WZ>::W, TbShape<T, R, WX, WZ>::H, launch_tb<T, R, WX, WZ>, false }
+// z-streaming configurations (ts.h): H rows per segment
 TSW_TS_CONFIGS_A(TSW_TS_EXTERN)
 TSW_TS_CONFIGS_B(TSW_TS_EXTERN)
-#define TS_CFG(T, R, WX, H) {T, R, WX, 1, TsShape<T, R, WX>::W, H, launch_ts<T, R, WX, H>},
+#define TS_CFG(T, WX, H) {T, 2, WX, T + 1, TsShape<T, WX>::W, H, launch_ts<T, WX, H>, true},
 static const TbCfg kTbCfgs[] = {
     TB_CFG(4, 8, 2, 8),  // 1: 128 x 64 extended, 112 x 48 core, 1024 threads
     TB_CFG(2, 8, 2, 4),  // 2: 128 x 32, 120 x 24 core
@@ -246,7 +247,7 @@ int tb_config(int id, int* T, int* W, int* H) {
 }
 
 int tb_max_sponge(int id) {
-  return kTbCfgs[id - 1].WZ == 1 ? kTsMaxSponge : 1 << 30;  // k_ts: WZ 1
+  return kTbCfgs[id - 1].streaming ? kTsMaxSponge : 1 << 30;  // k_ts keeps the table in LDS
 }
 
 void tb_launch(int id, const BatchConst& bc, const TbShot* shots, const TbEntry* ents,

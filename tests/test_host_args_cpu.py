@@ -88,12 +88,13 @@ def test_streaming_kernels_refuse_wide_sponges():
 
 
 def _wz1(L, i):
-    """Configuration i is a streaming one: one row of threads spans the strip
-    (core + 4T halo columns = threads), where a tile configuration has
-    several rows of threads."""
+    """Configuration i is a streaming one: its workgroup has one row of
+    threads across the strip (core + 4T halo columns) per sub-step plus one
+    that writes back, T + 1 rows, where no tile configuration has T + 1 rows
+    of threads."""
     import ctypes
 
     v = [ctypes.c_int() for _ in range(4)]
     L.tsw_kernel_config(i, *[ctypes.byref(x) for x in v])
     T, W, H, threads = (x.value for x in v)
-    return W + 4 * T == threads
+    return (W + 4 * T) * (T + 1) == threads
