@@ -13,6 +13,7 @@
 // device function.
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <vector>
 
@@ -165,10 +166,10 @@ struct Batch {
   int kernel = 0;
   std::vector<TbShot> tbs;
   struct TbLaunch {
-    int ent_off, pre_off, nent, ntiles;
+    int ent_off, tile_off, ntiles;
   };
   std::vector<TbLaunch> tbl;
-  DevBuf d_tbshots, d_tbent, d_tbpre;
+  DevBuf d_tbshots, d_tbent, d_tbtile;
   // caller's shot index -> lane (0 acoustic, 1 elastic) and index in the lane
   int ntotal = 0;
   std::vector<int> lane_of, local_of;
@@ -196,13 +197,13 @@ static void launch_step(Batch* B, const Epoch& e, int g, hipStream_t st) {
 
 // Instantiated configurations of the temporally blocked kernel
 // (T sub-steps, R rows per thread, WX waves across, WZ waves down).
-typedef void (*TbLaunchFn)(const BatchConst&, const TbShot*, const TbEntry*, const int*, int, int,
+typedef void (*TbLaunchFn)(const BatchConst&, const TbShot*, const TbEntry*, const TbTile*, int,
                            hipStream_t);
 template <int T, int R, int WX, int WZ>
 static void launch_tb(const BatchConst& bc, const TbShot* shots, const TbEntry* ents,
-                      const int* pre, int nent, int ntiles, hipStream_t st) {
+                      const TbTile* tiles, int ntiles, hipStream_t st) {
   hipLaunchKernelGGL((k_tb<T, R, WX, WZ>), dim3(ntiles), dim3(64 * WX, WZ), 0, st, bc, shots,
-                     ents, pre, nent, ntiles);
+                     ents, tiles);
 }
 struct TbCfg {
   int T, R, WX, WZ, W, H;  // k_ts: WZ = T + 1 wave rows (one per sub-step and the writer)
@@ -251,14 +252,14 @@ int tb_max_sponge(int id) {
 }
 
 void tb_launch(int id, const BatchConst& bc, const TbShot* shots, const TbEntry* ents,
-               const int* pre, int nent, int ntiles, hipStream_t st) {
-  kTbCfgs[id - 1].fn(bc, shots, ents, pre, nent, ntiles, st);
+               const TbTile* tiles, int ntiles, hipStream_t st) {
+  kTbCfgs[id - 1].fn(bc, shots, ents, tiles, ntiles, st);
 }
 
 static void launch_tb_rec(Batch* B, const Batch::TbLaunch& L, hipStream_t st) {
   const TbCfg& c = kTbCfgs[B->kernel - 1];
   c.fn(B->bc, (const TbShot*)B->d_tbshots.p, (const TbEntry*)B->d_tbent.p + L.ent_off,
-       B->d_tbpre.i() + L.pre_off, L.nent, L.ntiles, st);
+       (const TbTile*)B->d_tbtile.p + L.tile_off, L.ntiles, st);
 }
 
 // Longest-first list scheduling of shots onto slots; each shot gets one
@@ -396,13 +397,15 @@ static int build_tb(Batch* B) {
     P.illum = S.illum;
     P.image = S.image;
     tiles_x[i] = (S.nx + cf.W - 1) / cf.W;
-    tiles[i] = tiles_x[i] * ((S.nz + cf.H - 1) / cf.H);
+    const int tiles_z = (S.nz + cf.H - 1) / cf.H;
+    TSW_ARG(tiles_x[i] < (1 << 16) && tiles_z < (1 << 15), "shot %d: too many tiles for the blocked kernel", i);
+    tiles[i] = tiles_x[i] * tiles_z;
   }
   // per-slot block sequences
   std::vector<std::vector<TbEntry>> seq(B->slots);
   for (int s = 0; s < B->slots; ++s) {
     for (int id : slot_shots[s]) {
-      const int nt = B->hs[id].nt;
+      const int nt = B->hs[id].nt, k = B->sd[id].k;
       const int nb = (nt + T - 1) / T;
       for (int ph = 0; ph < 2; ++ph)
         for (int b = 0; b < nb; ++b) {
@@ -414,36 +417,50 @@ static int build_tb(Batch* B) {
           e.flags = (b == 0 ? kTbFirst : 0) | (b == nb - 1 ? kTbLast : 0);
           e.rd = b & 1;
           e.tiles_x = tiles_x[id];
+          tb_schedule(e, nt, k);
           seq[s].push_back(e);
         }
     }
   }
+  // Launch L runs the L-th block of every slot.  Its workgroup table lists
+  // (entry, tile) per workgroup in the XCD-aware order; it depends only on
+  // the launch's sequence of shots, so launches with the same shots share it.
   size_t nl = 0;
   for (auto& q : seq) nl = std::max(nl, q.size());
   std::vector<TbEntry> ents;
-  std::vector<int> pres;
+  std::vector<TbTile> tab;
+  std::map<std::vector<int>, int> tab_of;  // shots of a launch -> offset in `tab`
   for (size_t L = 0; L < nl; ++L) {
-    Batch::TbLaunch rec{(int)ents.size(), (int)pres.size(), 0, 0};
-    pres.push_back(0);
+    Batch::TbLaunch rec{(int)ents.size(), 0, 0};
+    std::vector<int> shots;
     for (int s = 0; s < B->slots; ++s) {
       if (L >= seq[s].size()) continue;
-      const TbEntry& e = seq[s][L];
-      ents.push_back(e);
-      pres.push_back(pres.back() + tiles[e.shot]);
-      rec.nent++;
+      ents.push_back(seq[s][L]);
+      shots.push_back(seq[s][L].shot);
     }
-    rec.ntiles = pres.back();
-    if (rec.nent) B->tbl.push_back(rec);
-    else pres.pop_back();
+    if (shots.empty()) continue;
+    std::vector<TbTile> lin;
+    for (int e = 0; e < (int)shots.size(); ++e) {
+      const int id = shots[e];
+      for (int t = 0; t < tiles[id]; ++t) lin.push_back({e, (t % tiles_x[id]) | (t / tiles_x[id]) << 16});
+    }
+    rec.ntiles = (int)lin.size();
+    auto it = tab_of.find(shots);
+    if (it == tab_of.end()) {
+      it = tab_of.emplace(shots, (int)tab.size()).first;
+      for (int b = 0; b < rec.ntiles; ++b) tab.push_back(lin[xcd_tile_tb(b, rec.ntiles)]);
+    }
+    rec.tile_off = it->second;
+    B->tbl.push_back(rec);
   }
   TSW_TRY(alloc(B, B->d_tbshots, B->tbs.size() * sizeof(TbShot)));
   TSW_TRY(alloc(B, B->d_tbent, std::max<size_t>(ents.size(), 1) * sizeof(TbEntry)));
-  TSW_TRY(alloc(B, B->d_tbpre, std::max<size_t>(pres.size(), 1) * sizeof(int)));
+  TSW_TRY(alloc(B, B->d_tbtile, std::max<size_t>(tab.size(), 1) * sizeof(TbTile)));
   TSW_HIP(hipMemcpy(B->d_tbshots.p, B->tbs.data(), B->tbs.size() * sizeof(TbShot),
                     hipMemcpyHostToDevice));
   TSW_HIP(hipMemcpy(B->d_tbent.p, ents.data(), ents.size() * sizeof(TbEntry),
                     hipMemcpyHostToDevice));
-  TSW_HIP(hipMemcpy(B->d_tbpre.p, pres.data(), pres.size() * sizeof(int), hipMemcpyHostToDevice));
+  TSW_HIP(hipMemcpy(B->d_tbtile.p, tab.data(), tab.size() * sizeof(TbTile), hipMemcpyHostToDevice));
   return kOk;
 }
 
@@ -500,7 +517,9 @@ int tsw_batch_create(int nshot, const tsw_shot_desc* shots, double dx, double dz
     for (int r = 0; r < d.nrec; ++r)
       TSW_ARG(d.rix[r] >= 0 && d.rix[r] < d.nx, "shot %d: receiver %d column %d outside [0, %d)",
               i, r, d.rix[r], d.nx);
-    TSW_ARG((long long)d.nx * d.nz < (1LL << 31), "shot %d: grid too large", i);
+    // every field stays under 4 GiB: the blocked kernels address it in bytes
+    // with 32-bit offsets
+    TSW_ARG((long long)d.nx * d.nz < (1LL << 30), "shot %d: grid too large", i);
     if (d.kind == TSW_SHOT_ACOUSTIC) {
       TSW_ARG(d.vel_fwd && d.vel_bwd, "shot %d: missing velocity models", i);
       lane_of[i] = 0;

@@ -69,7 +69,7 @@ struct Server {
   BatchConst bc{};
   EConst ec{};
   hipStream_t sa = nullptr, se = nullptr;
-  DevBuf d_table, d_tbshots, d_eshots, d_tbent, d_tbpre, d_elist;
+  DevBuf d_table, d_tbshots, d_eshots, d_tbent, d_tbtile, d_elist;
   Arena arena;
   void* arena_mem = nullptr;
   // resident models shared by requests (0/1 acoustic fwd/bwd velocity,
@@ -275,12 +275,12 @@ static int tick(Server* S, std::vector<Req*>& done, long long& cu, double& busy)
   // acoustic: blocked launches, each running request advances one block each
   const int nlaunch = std::max(1, S->tick_steps / S->T);
   std::vector<TbEntry> ents;
-  std::vector<int> pres;
-  std::vector<std::pair<int, int>> lr;  // (entry offset, prefix offset) per launch
-  std::vector<int> lnent, ltiles;
+  std::vector<TbTile> tab;              // per launch: (entry, tile) per workgroup
+  std::vector<std::pair<int, int>> lr;  // (entry offset, table offset) per launch
+  std::vector<int> ltiles;
   for (int L = 0; L < nlaunch && !act_a.empty(); ++L) {
-    const int eo = (int)ents.size(), po = (int)pres.size();
-    pres.push_back(0);
+    const int eo = (int)ents.size(), to = (int)tab.size();
+    std::vector<TbTile> lin;
     for (Req* r : act_a) {
       const int nb = (r->nt + S->T - 1) / S->T;
       if (r->cursor >= 2 * nb) continue;
@@ -294,20 +294,19 @@ static int tick(Server* S, std::vector<Req*>& done, long long& cu, double& busy)
       e.flags = (b == 0 ? kTbFirst : 0) | (b == nb - 1 ? kTbLast : 0);
       e.rd = b & 1;
       e.tiles_x = (r->nx + S->W - 1) / S->W;
+      tb_schedule(e, r->nt, S->k);
       const int tiles = e.tiles_x * ((r->nz + S->H - 1) / S->H);
+      const int ei = (int)ents.size() - eo;
+      for (int t = 0; t < tiles; ++t) lin.push_back({ei, (t % e.tiles_x) | (t / e.tiles_x) << 16});
       ents.push_back(e);
-      pres.push_back(pres.back() + tiles);
       r->cursor++;
       cu += (long long)e.nsub * r->cells();
     }
-    const int n = (int)ents.size() - eo;
-    if (n == 0) {
-      pres.pop_back();
-      break;
-    }
-    lr.push_back({eo, po});
-    lnent.push_back(n);
-    ltiles.push_back(pres.back());
+    if (lin.empty()) break;
+    const int nti = (int)lin.size();
+    for (int b = 0; b < nti; ++b) tab.push_back(lin[xcd_tile_tb(b, nti)]);
+    lr.push_back({eo, to});
+    ltiles.push_back(nti);
   }
   // elastic: per-step lists of running requests (ids are slots)
   std::vector<int> elist;
@@ -330,12 +329,12 @@ static int tick(Server* S, std::vector<Req*>& done, long long& cu, double& busy)
     elist.insert(elist.end(), pre.begin(), pre.end());
   }
   TSW_TRY(S->d_tbent.ensure(std::max<size_t>(ents.size(), 1) * sizeof(TbEntry)));
-  TSW_TRY(S->d_tbpre.ensure(std::max<size_t>(pres.size(), 1) * sizeof(int)));
+  TSW_TRY(S->d_tbtile.ensure(std::max<size_t>(tab.size(), 1) * sizeof(TbTile)));
   TSW_TRY(S->d_elist.ensure(std::max<size_t>(elist.size(), 1) * sizeof(int)));
   if (!ents.empty()) {
     TSW_HIP(hipMemcpyAsync(S->d_tbent.p, ents.data(), ents.size() * sizeof(TbEntry),
                            hipMemcpyHostToDevice, S->sa));
-    TSW_HIP(hipMemcpyAsync(S->d_tbpre.p, pres.data(), pres.size() * sizeof(int),
+    TSW_HIP(hipMemcpyAsync(S->d_tbtile.p, tab.data(), tab.size() * sizeof(TbTile),
                            hipMemcpyHostToDevice, S->sa));
   }
   if (!elist.empty())
@@ -349,8 +348,8 @@ static int tick(Server* S, std::vector<Req*>& done, long long& cu, double& busy)
   }
   for (size_t L = 0; L < lr.size(); ++L)
     tb_launch(S->kernel, S->bc, (const TbShot*)S->d_tbshots.p,
-              (const TbEntry*)S->d_tbent.p + lr[L].first, S->d_tbpre.i() + lr[L].second,
-              lnent[L], ltiles[L], S->sa);
+              (const TbEntry*)S->d_tbent.p + lr[L].first, (const TbTile*)S->d_tbtile.p + lr[L].second,
+              ltiles[L], S->sa);
   TSW_HIP(hipGetLastError());
   // completion: acoustic after its last block, elastic after step 2*nt-1
   const int g_end = S->g_e + S->tick_steps;
@@ -606,6 +605,11 @@ int tsw_server_submit(tsw_server_t h, const tsw_shot_desc* d, long long* id_out)
   TSW_ARG(S && d && id_out, "null argument");
   TSW_ARG(d->kind == TSW_SHOT_ACOUSTIC || d->kind == TSW_SHOT_ELASTIC, "bad kind");
   TSW_ARG(d->nx >= 5 && d->nz >= 5 && d->nt >= 1 && d->src, "bad request grid or source");
+  // fields stay under 4 GiB (32-bit byte offsets in the blocked kernel) and
+  // tile indices fit the kernel's 16-bit halves
+  TSW_ARG((long long)d->nx * d->nz < (1LL << 30) && (d->nx + S->W - 1) / S->W < (1 << 16) &&
+              (d->nz + S->H - 1) / S->H < (1 << 15),
+          "request grid %d x %d too large", d->nx, d->nz);
   TSW_ARG(d->six >= 0 && d->six < d->nx && d->siz >= 0 && d->siz < d->nz, "source outside grid");
   TSW_ARG(d->riz >= 0 && d->riz < d->nz && d->nrec >= 0, "bad receiver row");
   TSW_ARG(d->nrec == 0 || (d->rix && d->data), "receivers need rix and data");

@@ -50,46 +50,6 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
-// A pointer that is uniform over the wave, passed through readfirstlane so
-// that the compiler keeps it in SGPRs as an opaque base: every access is then
-// that scalar base plus one 32-bit vector byte offset (`at`), which global
-// memory instructions take directly.  Without it the compiler folds the lane
-// offset into 64-bit per-lane pointers and adds row offsets to those.
-template <typename P>
-__device__ __forceinline__ P* uniform_ptr(P* p) {
-  const unsigned long long a = (unsigned long long)p;
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
-  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
-  return (P*)((unsigned long long)hi << 32 | lo);
-}
-__device__ __forceinline__ const gfloat* at(const gfloat* base, unsigned boff) {
-  using C = const __attribute__((address_space(1))) char;
-  return (const gfloat*)((C*)base + boff);
-}
-__device__ __forceinline__ gfloat* at(gfloat* base, unsigned boff) {
-  using C = __attribute__((address_space(1))) char;
-  return (gfloat*)((C*)base + boff);
-}
-
-// `x` as a value the compiler cannot relate to earlier ones.  A load in a
-// conditional block then computes its 64-bit address in that block, where
-// instruction selection sees the scalar base plus the zero-extended 32-bit
-// offset and emits the SGPR-base form; with the extended offset inherited
-// from a dominating block it falls back to 64-bit vector addresses.
-__device__ __forceinline__ unsigned fresh(unsigned x) {
-  asm volatile("" : "+v"(x));
-  return x;
-}
-
-// `x` (uniform) as a value the compiler cannot relate to earlier ones: per
-// step of an unrolled loop this keeps it from hoisting one copy of every
-// step- and stage-dependent uniform expression out of the loop, which would
-// need more scalar registers than there are.
-__device__ __forceinline__ int opaque(int x) {
-  asm volatile("" : "+s"(x));
-  return x;
-}
-
 // Largest sponge strip the streaming kernel's LDS copy of the table holds.
 constexpr int kTsMaxSponge = 255;
 
@@ -527,19 +487,12 @@ __device__ __forceinline__ void ts_block(const BatchConst& bc, const TbShot& P, 
 
 template <int T, int WX, int H>
 __global__ __launch_bounds__(64 * WX * (T + 1)) void k_ts(BatchConst bc, const TbShot* shots,
-                                                    const TbEntry* ents, const int* pre,
-                                                    int nent, int ntiles) {
+                                                    const TbEntry* ents, const TbTile* tiles) {
   __shared__ TsLds<T, WX> sh;
-  const int b = xcd_tile_tb(blockIdx.x, ntiles);
-  int lo = 0, hi = nent - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (pre[mid] <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  const TbEntry E = ents[lo];
+  const TbTile tile = tiles[blockIdx.x];
+  const TbEntry E = ents[tile.ent];
   const TbShot& P = shots[E.shot];
-  const int t = b - pre[lo];
+  const int t = (tile.txz >> 16) * E.tiles_x + (tile.txz & 0xffff);
   if (E.phase == 0) {
     if (E.flags & kTbFirst) ts_block<T, WX, H, true, true, false>(bc, P, E, t, sh);
     else ts_block<T, WX, H, true, false, false>(bc, P, E, t, sh);
@@ -550,10 +503,10 @@ __global__ __launch_bounds__(64 * WX * (T + 1)) void k_ts(BatchConst bc, const T
 }
 
 template <int T, int WX, int H>
-void launch_ts(const BatchConst& bc, const TbShot* shots, const TbEntry* ents, const int* pre,
-               int nent, int ntiles, hipStream_t st) {
-  hipLaunchKernelGGL((k_ts<T, WX, H>), dim3(ntiles), dim3(64 * WX, T + 1), 0, st, bc, shots, ents, pre,
-                     nent, ntiles);
+void launch_ts(const BatchConst& bc, const TbShot* shots, const TbEntry* ents, const TbTile* tiles,
+               int ntiles, hipStream_t st) {
+  hipLaunchKernelGGL((k_ts<T, WX, H>), dim3(ntiles), dim3(64 * WX, T + 1), 0, st, bc, shots, ents,
+                     tiles);
 }
 
 // The instantiated streaming configurations (T sub-steps, WX waves across,
@@ -564,9 +517,9 @@ void launch_ts(const BatchConst& bc, const TbShot* shots, const TbEntry* ents, c
 
 #define TSW_TS_EXTERN(T, WX, H)                                                              \
   extern template void launch_ts<T, WX, H>(const BatchConst&, const TbShot*, const TbEntry*, \
-                                           const int*, int, int, hipStream_t);
+                                           const TbTile*, int, hipStream_t);
 #define TSW_TS_INSTANTIATE(T, WX, H)                                                  \
   template void launch_ts<T, WX, H>(const BatchConst&, const TbShot*, const TbEntry*, \
-                                    const int*, int, int, hipStream_t);
+                                    const TbTile*, int, hipStream_t);
 
 }  // namespace tsw

@@ -50,7 +50,11 @@ static void run_shot(const ShotSpec& sp, int fs, int T, int W, int H, Launch lau
   bc.inj = 0.25e-6f; bc.w = 1e-3f; bc.table = table.data();
   const int tiles_x = (nx + W - 1) / W, tiles = tiles_x * ((nz + H - 1) / H);
   const int nb = (nt + T - 1) / T;
-  int pre[2] = {0, tiles};
+  std::vector<TbTile> tab(tiles);
+  for (int b = 0; b < tiles; ++b) {
+    const int t = xcd_tile_tb(b, tiles);
+    tab[b] = {0, (t % tiles_x) | (t / tiles_x) << 16};
+  }
   for (int ph = 0; ph < 2; ++ph)
     for (int b = 0; b < nb; ++b) {
       TbEntry e{};
@@ -59,7 +63,8 @@ static void run_shot(const ShotSpec& sp, int fs, int T, int W, int H, Launch lau
       e.nsub = ph == 0 ? std::min(T, nt - e.it0) : std::min(T, e.it0 + 1);
       e.flags = (b == 0 ? kTbFirst : 0) | (b == nb - 1 ? kTbLast : 0);
       e.rd = b & 1; e.tiles_x = tiles_x;
-      launch(bc, &P, &e, pre, 1, tiles);
+      tb_schedule(e, nt, k);
+      launch(bc, &P, &e, tab.data(), tiles);
     }
 }
 
@@ -97,12 +102,12 @@ int main(int argc, char** argv) {
       }
       Arrays A, B;
       run_shot(sp, fs, 5, TbShape<5, 4, 1, 16>::W, TbShape<5, 4, 1, 16>::H,
-               [](const BatchConst& bc, const TbShot* s, const TbEntry* e, const int* pre, int ne, int nti) {
-                 hipLaunchKernelGGL((k_tb<5, 4, 1, 16>), dim3(nti), dim3(64, 16), 0, 0, bc, s, e, pre, ne, nti);
+               [](const BatchConst& bc, const TbShot* s, const TbEntry* e, const TbTile* tb, int nti) {
+                 hipLaunchKernelGGL((k_tb<5, 4, 1, 16>), dim3(nti), dim3(64, 16), 0, 0, bc, s, e, tb);
                }, A, coef_f, coef_b, src, data, roff, ridx, table, ns, nrec);
       run_shot(sp, fs, TS_T, TsShape<TS_T, TS_WX>::W, TS_H,
-               [](const BatchConst& bc, const TbShot* s, const TbEntry* e, const int* pre, int ne, int nti) {
-                 launch_ts<TS_T, TS_WX, TS_H>(bc, s, e, pre, ne, nti, nullptr);
+               [](const BatchConst& bc, const TbShot* s, const TbEntry* e, const TbTile* tb, int nti) {
+                 launch_ts<TS_T, TS_WX, TS_H>(bc, s, e, tb, nti, nullptr);
                }, B, coef_f, coef_b, src, data, roff, ridx, table, ns, nrec);
       auto cmp = [&](const char* what, const std::vector<float>& a, const std::vector<float>& b) {
         long long bad = 0, first = -1;
