@@ -21,6 +21,7 @@
 #include "elabatch.h"
 #include "post.h"
 #include "tb.h"
+#include "tc.h"
 #include "ts.h"
 #include "tswrtm.h"
 
@@ -216,6 +217,7 @@ struct TbCfg {
 TSW_TS_CONFIGS_A(TSW_TS_EXTERN)
 TSW_TS_CONFIGS_B(TSW_TS_EXTERN)
 #define TS_CFG(T, WX, H) {T, 2, WX, T + 1, TsShape<T, WX>::W, H, launch_ts<T, WX, H>, true},
+#define TC_CFG(T, R) {T, R, 1, 16, TcShape<T, R>::W, TcShape<T, R>::H, launch_tc<T, R>, false},
 static const TbCfg kTbCfgs[] = {
     TB_CFG(4, 8, 2, 8),  // 1: 128 x 64 extended, 112 x 48 core, 1024 threads
     TB_CFG(2, 8, 2, 4),  // 2: 128 x 32, 120 x 24 core
@@ -235,6 +237,11 @@ static const TbCfg kTbCfgs[] = {
     TB_CFG(6, 4, 1, 16),  // 16: 64 x 64, 40 x 40 core
     // 17..24: z-streaming strips of 64 WX columns, depth segments of H rows
     TSW_TS_CONFIGS_A(TS_CFG) TSW_TS_CONFIGS_B(TS_CFG)
+    // 25..28: time levels in LDS (tc.h), 64 x 16R extended, 1024 threads
+    TC_CFG(5, 6)  // 25: 64 x 96, 44 x 76 core
+    TC_CFG(4, 6)  // 26: 64 x 96, 48 x 80 core
+    TC_CFG(5, 5)  // 27: 64 x 80, 44 x 60 core
+    TC_CFG(6, 6)  // 28: 64 x 96, 40 x 72 core
 };
 constexpr int kNumTbCfgs = (int)(sizeof(kTbCfgs) / sizeof(kTbCfgs[0]));
 

@@ -1,9 +1,11 @@
-// Runs the streaming blocked kernel (k_ts, das_amd/csrc/ts.h) and the tiled
-// blocked kernel (k_tb, das_amd/csrc/tb.h) on the host, through the CPU
+// Runs the streaming blocked kernel (k_ts, das_amd/csrc/ts.h), the tiled
+// blocked kernel (k_tb, das_amd/csrc/tb.h) and, built with -DTC_T=.. -DTC_R=..,
+// the blocked kernel with its levels in LDS (k_tc, das_amd/csrc/tc.h) on the
+// host, through the CPU
 // stand-in for the HIP runtime in tests/cpp/hip_emu, over every block of both
 // phases of each shot, and compares illumination, image and every snapshot
 // bit for bit.  k_tb is bit-identical to the per-step kernel and the oracle on
-// the GPU; this pins k_ts to it on the CPU for shapes that cover one and
+// the GPU; this pins k_ts (and k_tc) to it on the CPU for shapes that cover one and
 // several depth segments and strips, edge and duplicate receivers, sources
 // and receivers on rows 0..5, the free surface on and off, and blocks of fewer
 // than T sub-steps.  Inputs are seeded random values (models, wavelet, data,
@@ -14,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <random>
+#include "tc.h"
 #include "ts.h"
 
 using namespace tsw;
@@ -109,6 +112,13 @@ int main(int argc, char** argv) {
                [](const BatchConst& bc, const TbShot* s, const TbEntry* e, const TbTile* tb, int nti) {
                  launch_ts<TS_T, TS_WX, TS_H>(bc, s, e, tb, nti, nullptr);
                }, B, coef_f, coef_b, src, data, roff, ridx, table, ns, nrec);
+#ifdef TC_T
+      Arrays C;
+      run_shot(sp, fs, TC_T, TcShape<TC_T, TC_R>::W, TcShape<TC_T, TC_R>::H,
+               [](const BatchConst& bc, const TbShot* s, const TbEntry* e, const TbTile* tb, int nti) {
+                 launch_tc<TC_T, TC_R>(bc, s, e, tb, nti, nullptr);
+               }, C, coef_f, coef_b, src, data, roff, ridx, table, ns, nrec);
+#endif
       auto cmp = [&](const char* what, const std::vector<float>& a, const std::vector<float>& b) {
         long long bad = 0, first = -1;
         for (size_t i = 0; i < a.size(); ++i)
@@ -124,6 +134,11 @@ int main(int argc, char** argv) {
       cmp("illum", A.illum, B.illum);
       cmp("image", A.image, B.image);
       cmp("snaps", A.snaps, B.snaps);
+#ifdef TC_T
+      cmp("illum (k_tc)", A.illum, C.illum);
+      cmp("image (k_tc)", A.image, C.image);
+      cmp("snaps (k_tc)", A.snaps, C.snaps);
+#endif
       std::fflush(stdout);
     }
   std::printf("%s (%lld barriers)\n", fails ? "FAIL" : "OK", emu::barriers());
