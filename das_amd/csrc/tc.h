@@ -47,6 +47,14 @@ struct TcShape {
   static_assert(R >= 3, "rows 0..2 of a free surface must belong to one thread");
 };
 
+using lfloat = __attribute__((address_space(3))) float;
+// `p` as a value the compiler takes as given: it keeps one register for it
+// instead of folding each access's constant into a new address
+__device__ __forceinline__ const lfloat* lds_reg(const lfloat* p) {
+  asm volatile("" : "+v"(p));
+  return p;
+}
+
 template <int T, int R>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_tc(
     BatchConst bc, const TbShot* shots, const TbEntry* ents, const TbTile* tiles) {
@@ -267,10 +275,15 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
     // leapfrog update, evaluated on every lane
     const float* const X = TC_LEVEL(j);
     float* const Y = TC_LEVEL(j + 1);
+    // row bases for the LDS reads: rows -2..1 from Xa, 2..5 from Xb, 6.. from
+    // Xc, so each row's reads are small positive offsets of one register
+    const lfloat* const Xa = lds_reg((const lfloat*)(X - 2 * S::CX - 2));
+    const lfloat* const Xb = lds_reg((const lfloat*)(X + 2 * S::CX - 2));
+    const lfloat* const Xc = lds_reg((const lfloat*)(X + 6 * S::CX - 2));
     float n[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      const float* const xr = X + q * S::CX;
+      const lfloat* const xr = (q < 2 ? Xa + (q + 2) * S::CX : q < 6 ? Xb + (q - 2) * S::CX : Xc + (q - 6) * S::CX) + 2;
       const float pc = c[q];
       const float xm2 = xr[-2], xm1 = xr[-1], xp1 = xr[1], xp2 = xr[2];
       const float zm2 = q >= 2 ? c[q - 2] : xr[-2 * S::CX];
