@@ -714,16 +714,18 @@ def main():
     shots_per_s = cfg.n_shots * args.steps / elapsed
 
     kernel_ms, launches = batch.time_kernels()
-    from das_amd.batch import kernel_configs
+    from das_amd.batch import kernel_configs, kernel_name
 
     T = kernel_configs()[args.kernel][0] if args.kernel > 0 else 1
+    kname = kernel_name(args.kernel) if args.kernel > 0 else "k_batch_step"
+    family = kname.split("<")[0]
     if args.kernel > 0:
         alg = blocked_algorithmic_bytes(cfg.shots, T, cfg.k)
     else:
         alg = BYTES_PER_CELL_UPDATE * cu
     achieved = alg / (kernel_ms * 1e-3) / 1e9
     effective = BYTES_PER_CELL_UPDATE * cu / (kernel_ms * 1e-3) / 1e9
-    counted = counted_traffic("k_tb_hbm_traffic.json") if args.kernel > 0 else None
+    counted = counted_traffic(f"{family}_hbm_traffic.json") if args.kernel > 0 else None
     if counted is not None and counted.get("kernel_config") != args.kernel:
         counted = None
     # HBM bytes per launch: the blocked scheme's algorithmic bytes times the
@@ -746,7 +748,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": (f"k_tb configuration {args.kernel} (T={T})" if args.kernel
+                     "kernel": (f"{kname}, configuration {args.kernel} (T={T})" if args.kernel
                                 else "k_batch_step"),
                      "launches": launches,
                      "avg_launch_us": round(kernel_ms * 1e3 / max(launches, 1), 3),
@@ -769,7 +771,7 @@ def main():
             "bytes_per_cell_update": round(traffic * launches / cu, 3),
             "counted_over_algorithmic": counted["counted_over_algorithmic"],
             "gbs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
-            "source": "profiles/k_tb_hbm_traffic.json (rocprofv3 FETCH_SIZE, WRITE_SIZE, "
+            "source": f"profiles/{family}_hbm_traffic.json (rocprofv3 FETCH_SIZE, WRITE_SIZE, "
                       "copy-kernel calibrated)"}
     if args.kernel > 0 and not args.no_check:
         out["check"] = check_against_per_step(cfg, batch, (0, cfg.n_shots - 1), log)

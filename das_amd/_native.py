@@ -48,6 +48,7 @@ SIGNATURES = {
     "tsw_batch_create": (I, [I, P, D, D, D, I, D, I, I, I, I, I, ctypes.POINTER(P)]),
     "tsw_kernel_configs": (I, []),
     "tsw_kernel_config": (I, [I] + [ctypes.POINTER(I)] * 4),
+    "tsw_kernel_name": (I, [I, ctypes.c_char_p, I]),
     "tsw_batch_run": (I, [P, I]),
     "tsw_batch_time_kernels": (I, [P, I, ctypes.POINTER(D), ctypes.POINTER(I)]),
     "tsw_batch_shot_result": (I, [P, I, P, P]),

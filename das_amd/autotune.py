@@ -20,7 +20,7 @@ import numpy as np
 from . import _native as N
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
-DEFAULT_KERNELS = (0, 7, 9, 10, 11, 12, 15, 16)
+DEFAULT_KERNELS = (0, 7, 9, 10, 11, 12, 15, 16, 25, 26, 27, 28)
 
 
 def device_name() -> str:

@@ -65,6 +65,14 @@ def kernel_configs():
     return out
 
 
+def kernel_name(kid: int) -> str:
+    """Kernel and template arguments of blocked configuration `kid`, for
+    example "k_tc<5,6>"."""
+    buf = ctypes.create_string_buffer(64)
+    N.check(N.lib().tsw_kernel_name(kid, buf, 64))
+    return buf.value.decode()
+
+
 class ShotBatch:
     """A resident batch of shots: objects with the fields of workloads.Shot
     (acoustic) or workloads.ElasticShot (kind == "elastic")."""

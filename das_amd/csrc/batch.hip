@@ -210,14 +210,15 @@ struct TbCfg {
   int T, R, WX, WZ, W, H;  // k_ts: WZ = T + 1 wave rows (one per sub-step and the writer)
   TbLaunchFn fn;
   bool streaming;
+  bool tc;  // time levels in LDS (tc.h)
 };
 #define TB_CFG(T, R, WX, WZ) \
-  { T, R, WX, WZ, TbShape<T, R, WX, WZ>::W, TbShape<T, R, WX, WZ>::H, launch_tb<T, R, WX, WZ>, false }
+  { T, R, WX, WZ, TbShape<T, R, WX, WZ>::W, TbShape<T, R, WX, WZ>::H, launch_tb<T, R, WX, WZ>, false, false }
 // z-streaming configurations (ts.h): H rows per segment
 TSW_TS_CONFIGS_A(TSW_TS_EXTERN)
 TSW_TS_CONFIGS_B(TSW_TS_EXTERN)
-#define TS_CFG(T, WX, H) {T, 2, WX, T + 1, TsShape<T, WX>::W, H, launch_ts<T, WX, H>, true},
-#define TC_CFG(T, R) {T, R, 1, 16, TcShape<T, R>::W, TcShape<T, R>::H, launch_tc<T, R>, false},
+#define TS_CFG(T, WX, H) {T, 2, WX, T + 1, TsShape<T, WX>::W, H, launch_ts<T, WX, H>, true, false},
+#define TC_CFG(T, R) {T, R, 1, 16, TcShape<T, R>::W, TcShape<T, R>::H, launch_tc<T, R>, false, true},
 static const TbCfg kTbCfgs[] = {
     TB_CFG(4, 8, 2, 8),  // 1: 128 x 64 extended, 112 x 48 core, 1024 threads
     TB_CFG(2, 8, 2, 4),  // 2: 128 x 32, 120 x 24 core
@@ -899,6 +900,16 @@ int tsw_batch_info(tsw_batch_t h, long long* cell_updates, long long* device_byt
 }
 
 int tsw_kernel_configs(void) { return kNumTbCfgs; }
+
+int tsw_kernel_name(int id, char* buf, int len) {
+  TSW_ARG(id >= 1 && id <= kNumTbCfgs, "kernel config %d outside [1, %d]", id, kNumTbCfgs);
+  TSW_ARG(buf && len > 0, "null buffer");
+  const TbCfg& c = kTbCfgs[id - 1];
+  if (c.streaming) snprintf(buf, len, "k_ts<%d,%d,%d>", c.T, c.WX, c.H);
+  else if (c.tc) snprintf(buf, len, "k_tc<%d,%d>", c.T, c.R);
+  else snprintf(buf, len, "k_tb<%d,%d,%d,%d>", c.T, c.R, c.WX, c.WZ);
+  return kOk;
+}
 
 int tsw_kernel_config(int id, int* T, int* W, int* H, int* threads) {
   TSW_ARG(id >= 1 && id <= kNumTbCfgs, "kernel config %d outside [1, %d]", id, kNumTbCfgs);

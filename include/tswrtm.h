@@ -203,6 +203,9 @@ int tsw_batch_create(int nshot, const tsw_shot_desc* shots, double dx, double dz
  * `id`'s steps per launch, core tile (W columns x H rows) and threads. */
 int tsw_kernel_configs(void);
 int tsw_kernel_config(int id, int* T, int* W, int* H, int* threads);
+/* Configuration `id`'s kernel with its template arguments (for example
+ * "k_tc<5,6>"), NUL-terminated into `buf` of `len` bytes. */
+int tsw_kernel_name(int id, char* buf, int len);
 /* Migrates every shot (mode TSW_RUN_BATCHED or TSW_RUN_PLAIN); synchronous. */
 int tsw_batch_run(tsw_batch_t batch, int mode);
 /* Runs one lane of the batch (0 acoustic, 1 elastic) once with a HIP event
