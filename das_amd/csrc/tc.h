@@ -11,11 +11,12 @@
 // level j+1 over level j-2.  Cells outside [2,nz-2) x [2,nx-2) keep the value
 // of three levels back (wave2d.c:19-82), which is what their slot of the
 // target buffer already holds, so no thread keeps that level in registers;
-// registers hold the current and previous level of the thread's cells, the
-// model coefficient, the sponge factor and the image / illumination
-// accumulator.  One barrier per sub-step; after s sub-steps the cells whose
-// distance to a cut (non-grid) tile edge is >= 2s are exact, so after T
-// sub-steps the core is exact and is written back.
+// registers hold the current level of the thread's cells, the model
+// coefficient, the sponge factor and the image / illumination accumulator,
+// and the previous level is read back from the thread's own LDS slots.  One
+// barrier per sub-step; after s sub-steps the cells whose distance to a cut
+// (non-grid) tile edge is >= 2s are exact, so after T sub-steps the core is
+// exact and is written back.
 //
 // Per sub-step the kernel reproduces the reference loop exactly
 // (tsw_rtm/wave_solver.py:157-183 forward, :210-242 backward): snapshot +
@@ -211,15 +212,16 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
     rsel = r1 - r0 == 1 ? as_global(P.ridx)[r0] : (r1 > r0 ? -2 : -1);
   }
 
-  // levels 0 (every cell), -1 and -2 (the kept cells read them back)
+  // levels 0 and -1 (every cell; a sub-step reads level j-1 of its own
+  // cells back) and -2 (the kept cells read it back)
 #pragma unroll
-  for (int q = 0; q < R; ++q) TC_LEVEL(0)[q * S::CX] = c[q];
+  for (int q = 0; q < R; ++q) {
+    TC_LEVEL(0)[q * S::CX] = c[q];
+    TC_LEVEL(-1)[q * S::CX] = bl[q];
+  }
   if (edge) {
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      TC_LEVEL(-1)[q * S::CX] = bl[q];
-      TC_LEVEL(-2)[q * S::CX] = a0[q];
-    }
+    for (int q = 0; q < R; ++q) TC_LEVEL(-2)[q * S::CX] = a0[q];
   }
   // the samples the injecting wave adds after each sub-step, in LDS slots of
   // its own lanes: no memory latency inside the sub-steps
@@ -240,6 +242,18 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
   float* snaps = P.snaps;
   const long long ncell = (long long)nx * nz;
   const float w = bc.w;
+  // Back-propagation reads a snapshot level every k sub-steps: its rows are
+  // loaded one sub-step ahead (the first in the prologue), so that the
+  // imaging sub-step does not wait on HBM.
+  float sv[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) sv[q] = 0.0f;
+  if (!fwd && (snap & 1)) {
+    const float* const ps = snaps + (long long)lvl * ncell;
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+      if (TC_CORE(q)) sv[q] = ld_at_nt(ps, lane4, roff[q]);
+  }
   // The sub-steps are unrolled (at most T of them), so the rotation of the
   // register levels is a renaming and each buffer's address is fixed.
 #pragma unroll
@@ -263,17 +277,21 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
           }
         }
       } else {
-        // every row's load is issued before the first use
-        float sv[R];
-#pragma unroll
-        for (int q = 0; q < R; ++q) sv[q] = TC_CORE(q) ? ld_at_nt(ps, lane4, roff[q]) : 0.0f;
 #pragma unroll
         for (int q = 0; q < R; ++q)
           if (TC_CORE(q)) acc[q] = acc[q] + (sv[q] * c[q]) * w;
       }
     }
+    if (!fwd && (snap >> (j + 1) & 1) && j + 1 < nsub) {
+      // the next imaging sub-step's snapshot rows (level `lvl` by now)
+      const float* const ps = snaps + (long long)lvl * ncell;
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+        if (TC_CORE(q)) sv[q] = ld_at_nt(ps, lane4, roff[q]);
+    }
     // leapfrog update, evaluated on every lane
     const float* const X = TC_LEVEL(j);
+    const float* const Xp = TC_LEVEL(j - 1);  // level j-1 of the thread's own cells
     float* const Y = TC_LEVEL(j + 1);
     // row bases for the LDS reads: rows -2..1 from Xa, 2..5 from Xb, 6.. from
     // Xc, so each row's reads are small positive offsets of one register
@@ -296,7 +314,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
       float lz2 = TSW_C2 * (zm2 + zp2);
       lz2 = lz2 + TSW_C1 * (zm1 + zp1);
       lz2 = (lz2 + TSW_C0 * pc) * bc.rdz2;
-      float v = (2.0f * pc - bl[q]) + cf[q] * (lx2 + lz2);
+      float v = (2.0f * pc - Xp[q * S::CX]) + cf[q] * (lx2 + lz2);
       n[q] = v * sp[q];
     }
     if (edge) {
@@ -339,26 +357,26 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       Y[q * S::CX] = n[q];
-      bl[q] = c[q];
       c[q] = n[q];
     }
     __syncthreads();
   }
 
-  // ---- write back the core: levels nsub, nsub-1 from registers, and level
-  // nsub-2 of kept cells from their own LDS slot
+  // ---- write back the core: level nsub from registers, level nsub-1 and
+  // (kept cells) nsub-2 from the thread's own LDS slots
   const int wr = rd ^ 1;
   float* const pacc = fwd ? P.illum : P.image;
   float* const pw2 = P.set[wr][2];
   float* const pw1 = P.set[wr][1];
   float* const pw0 = P.set[wr][0];
   const float* const Ak = TC_LEVEL(nsub - 2);
+  const float* const Bk = TC_LEVEL(nsub - 1);
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     if (TC_CORE(q)) {
       if (!last) {
         st_at(c[q], pw2, lane4, roff[q]);
-        st_at(bl[q], pw1, lane4, roff[q]);
+        st_at(Bk[q * S::CX], pw1, lane4, roff[q]);
         if (TC_KEEP(q)) st_at(Ak[q * S::CX], pw0, lane4, roff[q]);
       }
       st_at((fwd && last) ? acc[q] * w : acc[q], pacc, lane4, roff[q]);
