@@ -94,7 +94,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
   }
   // rows 0/1 under a free surface are rewritten every step, never "kept"
   const bool fs0 = bc.fs && row0 == 0;
-  const unsigned m_keep = m_in & ~m_int & ~(fs0 ? 3u : 0u);
+  unsigned m_keep = m_in & ~m_int & ~(fs0 ? 3u : 0u);
   // whether the wave holds any cell outside [2,nz-2) x [2,nx-2) (free-surface
   // rows and cells past the grid included); uniform
   const bool edge = !(X0 >= 2 && X0 + S::CX <= nx - 2 && row0 >= 2 && row0 + R <= nz - 2);
@@ -259,6 +259,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
 #pragma unroll
   for (int j = 0; j < T; ++j) {
     if (j == nsub) break;
+    // the row masks as values the compiler cannot carry from one sub-step to
+    // the next: each sub-step tests their bits again instead of holding six
+    // rows' lane masks in scalar registers through the whole loop
+    asm volatile("" : "+v"(m_int), "+v"(m_core), "+v"(m_keep));
     if (j == jend) {
       for (int r = j; r < nsub; ++r) __syncthreads();
       return;
