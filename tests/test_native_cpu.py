@@ -57,3 +57,38 @@ def test_recv_map_error_does_not_need_gpu():
         _norm_indices(np.array([10]), 10, "rix")
     with pytest.raises(IndexError):
         _norm_index(-11, 10, "six")
+
+
+def test_kernel_table_names_and_shapes():
+    """The blocked-kernel table (no device call): each configuration's kernel
+    family and template arguments agree with its steps, core tile and
+    threads; k_tc shapes are 64 - 4T columns by 16R - 4T rows."""
+    import re
+
+    from das_amd.batch import kernel_configs, kernel_name
+
+    cfgs = kernel_configs()
+    fams = {}
+    for kid, (T, W, H, threads) in cfgs.items():
+        name = kernel_name(kid)
+        m = re.fullmatch(r"(k_t[bsc])<([0-9,]+)>", name)
+        assert m, name
+        args = [int(v) for v in m.group(2).split(",")]
+        fams.setdefault(m.group(1), []).append(kid)
+        assert args[0] == T, (kid, name)
+        if m.group(1) == "k_tc":
+            R = args[1]
+            assert (W, H, threads) == (64 - 4 * T, 16 * R - 4 * T, 1024), (kid, name)
+        elif m.group(1) == "k_tb":
+            _, R, WX, WZ = args
+            assert (W, H, threads) == (64 * WX - 4 * T, WZ * R - 4 * T, 64 * WX * WZ), (kid, name)
+        else:
+            _, WX, Hs = args
+            assert (W, H, threads) == (64 * WX - 4 * T, Hs, 64 * WX * (T + 1)), (kid, name)
+    assert fams["k_tb"] == list(range(1, 17))
+    assert fams["k_ts"] == list(range(17, 25))
+    assert fams["k_tc"] == list(range(25, 29))
+    L = N.lib()
+    buf = ctypes.create_string_buffer(8)
+    assert L.tsw_kernel_name(0, buf, 8) == -1
+    assert L.tsw_kernel_name(len(cfgs) + 1, buf, 8) == -1
