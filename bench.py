@@ -624,7 +624,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--slots", type=int, default=-1, help="-1: chosen by the autotuner")
-    ap.add_argument("--graph-steps", type=int, default=100)
+    ap.add_argument("--graph-steps", type=int, default=20000,
+                    help="time steps per hipGraph of the batched run (20000: the whole "
+                         "workload in one graph, whose launch groups never join midway)")
     ap.add_argument("--kernel", type=int, default=-1,
                     help="-1: autotuned; 0: one launch per time step; c >= 1: temporally "
                          "blocked config c")

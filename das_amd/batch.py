@@ -107,8 +107,13 @@ class ShotBatch:
         N.check(N.lib().tsw_batch_run(self._h, m))
 
     def time_kernels(self, lane: str = "acoustic"):
-        """One run of a lane with a HIP event pair around each run of 32
-        back-to-back launches -> (summed milliseconds, launches)."""
+        """One run of a lane, timed by HIP events -> (milliseconds, launches).
+
+        A temporally blocked batch with hipGraphs is timed as it runs: its
+        graphs (each with the launch groups as parallel branches) inside one
+        event pair; otherwise one event pair goes around each run of 32
+        back-to-back launches.  A blocked launch is one block of T steps of
+        every active shot."""
         ms, n = N.D(), N.I()
         N.check(N.lib().tsw_batch_time_kernels(self._h, 1 if lane == "elastic" else 0,
                                                ctypes.byref(ms), ctypes.byref(n)))

@@ -7,11 +7,15 @@
 // covers the tiles of every shot active at that step (forward, back-
 // propagation or initialisation), so one launch advances the whole batch.
 // Launches are captured into hipGraphs of `graph_steps` steps and replayed.
+// The temporally blocked kernels split the slots into launch groups, each
+// with its own launch list and stream, so that the groups' launch boundaries
+// do not coincide (see `kLaunchGroups`).
 //
 // Per-shot results are bit-identical to the single-shot path
 // (tsw_rtm/rtm_engine.py:49-104 semantics): the cell update is the same
 // device function.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -29,6 +33,22 @@ namespace tsw {
 
 constexpr int kBX = 64;
 constexpr int kBZ = 4;
+
+// Launch groups of the temporally blocked kernels.  A launch ends with a tail
+// in which fewer and fewer workgroups run, and the next launch on the stream
+// starts with every workgroup in its load phase at once; with the slots in
+// several groups on their own streams, one group's launches fill the others'
+// ends.  On the batched 64-window workload (k_tc<5,6>) a step took 390 ms in
+// one group and 372, 375, 364 and 368 ms in 2, 3, 8 and 16, results
+// bit-identical.  The environment variable TSW_LAUNCH_GROUPS (1..64) sets
+// another number (1: every launch on the batch's stream).
+constexpr int kLaunchGroups = 8;
+static int launch_groups() {
+  const char* v = std::getenv("TSW_LAUNCH_GROUPS");
+  if (!v || !*v) return kLaunchGroups;
+  const int n = std::atoi(v);
+  return n >= 1 && n <= 64 ? n : kLaunchGroups;
+}
 
 // Blocks are dispatched to the 8 XCDs round-robin; map them so that each XCD
 // works on a contiguous range of tiles (neighbouring tiles share halos in
@@ -169,7 +189,14 @@ struct Batch {
   struct TbLaunch {
     int ent_off, tile_off, ntiles;
   };
-  std::vector<TbLaunch> tbl;
+  // launch groups: slot s belongs to group s % groups; group q's launch L runs
+  // the L-th block of each of its slots, on stream gstream[q] (group 0 on
+  // `stream`), forked from `stream` and joined back into it by issue()
+  int groups = 1;
+  std::vector<std::vector<TbLaunch>> tbl;
+  std::vector<hipStream_t> gstream;
+  hipEvent_t ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_join;
   DevBuf d_tbshots, d_tbent, d_tbtile;
   // caller's shot index -> lane (0 acoustic, 1 elastic) and index in the lane
   int ntotal = 0;
@@ -181,6 +208,11 @@ struct Batch {
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto e : graphs) (void)hipGraphExecDestroy(e);
     for (auto gdef : graph_defs) (void)hipGraphDestroy(gdef);
+    for (size_t q = 1; q < gstream.size(); ++q)
+      if (gstream[q]) (void)hipStreamDestroy(gstream[q]);
+    for (auto e : ev_join)
+      if (e) (void)hipEventDestroy(e);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -333,6 +365,10 @@ static void issue(Batch* B, int g0, int g1, hipStream_t st);
 static int launches_total(const Batch* B);
 
 // hipGraphs of `graph_steps` time steps each (graph_steps/T blocked launches).
+// A blocked graph holds every launch group's launches as parallel branches
+// between a fork and a join (captured from issue()); on the batched 64-window
+// workload that ran at 365 ms per step against 384 ms with each group's
+// launches in graphs of their own, replayed on the group's stream.
 static int capture_graphs(Batch* B) {
   const int total = launches_total(B);
   const int per = B->kernel > 0
@@ -344,11 +380,16 @@ static int capture_graphs(Batch* B) {
     issue(B, g0, g1, B->stream);
     hipGraph_t gr;
     TSW_HIP(hipStreamEndCapture(B->stream, &gr));
+    B->graph_defs.push_back(gr);
     hipGraphExec_t ex;
     TSW_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
-    B->graph_defs.push_back(gr);
     B->graphs.push_back(ex);
   }
+  return kOk;
+}
+
+static int replay_graphs(Batch* B, hipStream_t st) {
+  for (auto ex : B->graphs) TSW_HIP(hipGraphLaunch(ex, st));
   return kOk;
 }
 
@@ -430,7 +471,8 @@ static int build_tb(Batch* B) {
         }
     }
   }
-  // Launch L runs the L-th block of every slot.  Its workgroup table lists
+  // Launch L of a group runs the L-th block of each of the group's slots.  Its
+  // workgroup table lists
   // (entry, tile) per workgroup in the XCD-aware order; it depends only on
   // the launch's sequence of shots, so launches with the same shots share it.
   size_t nl = 0;
@@ -438,29 +480,31 @@ static int build_tb(Batch* B) {
   std::vector<TbEntry> ents;
   std::vector<TbTile> tab;
   std::map<std::vector<int>, int> tab_of;  // shots of a launch -> offset in `tab`
-  for (size_t L = 0; L < nl; ++L) {
-    Batch::TbLaunch rec{(int)ents.size(), 0, 0};
-    std::vector<int> shots;
-    for (int s = 0; s < B->slots; ++s) {
-      if (L >= seq[s].size()) continue;
-      ents.push_back(seq[s][L]);
-      shots.push_back(seq[s][L].shot);
+  B->tbl.assign(B->groups, {});
+  for (size_t L = 0; L < nl; ++L)
+    for (int grp = 0; grp < B->groups; ++grp) {
+      Batch::TbLaunch rec{(int)ents.size(), 0, 0};
+      std::vector<int> shots;
+      for (int s = grp; s < B->slots; s += B->groups) {
+        if (L >= seq[s].size()) continue;
+        ents.push_back(seq[s][L]);
+        shots.push_back(seq[s][L].shot);
+      }
+      if (shots.empty()) continue;  // the group's slots are all done
+      std::vector<TbTile> lin;
+      for (int e = 0; e < (int)shots.size(); ++e) {
+        const int id = shots[e];
+        for (int t = 0; t < tiles[id]; ++t) lin.push_back({e, (t % tiles_x[id]) | (t / tiles_x[id]) << 16});
+      }
+      rec.ntiles = (int)lin.size();
+      auto it = tab_of.find(shots);
+      if (it == tab_of.end()) {
+        it = tab_of.emplace(shots, (int)tab.size()).first;
+        for (int b = 0; b < rec.ntiles; ++b) tab.push_back(lin[xcd_tile_tb(b, rec.ntiles)]);
+      }
+      rec.tile_off = it->second;
+      B->tbl[grp].push_back(rec);
     }
-    if (shots.empty()) continue;
-    std::vector<TbTile> lin;
-    for (int e = 0; e < (int)shots.size(); ++e) {
-      const int id = shots[e];
-      for (int t = 0; t < tiles[id]; ++t) lin.push_back({e, (t % tiles_x[id]) | (t / tiles_x[id]) << 16});
-    }
-    rec.ntiles = (int)lin.size();
-    auto it = tab_of.find(shots);
-    if (it == tab_of.end()) {
-      it = tab_of.emplace(shots, (int)tab.size()).first;
-      for (int b = 0; b < rec.ntiles; ++b) tab.push_back(lin[xcd_tile_tb(b, rec.ntiles)]);
-    }
-    rec.tile_off = it->second;
-    B->tbl.push_back(rec);
-  }
   TSW_TRY(alloc(B, B->d_tbshots, B->tbs.size() * sizeof(TbShot)));
   TSW_TRY(alloc(B, B->d_tbent, std::max<size_t>(ents.size(), 1) * sizeof(TbEntry)));
   TSW_TRY(alloc(B, B->d_tbtile, std::max<size_t>(tab.size(), 1) * sizeof(TbTile)));
@@ -474,17 +518,34 @@ static int build_tb(Batch* B) {
 
 // Issues every launch of a batched run (steps [g0, g1) of the per-step
 // kernel, or launches [g0, g1) of the blocked kernel).
+// A launch of the blocked kernel is the L-th block of every slot: one kernel
+// per launch group, the groups on their own streams between a fork from `st`
+// and a join back into it.
 static int launches_total(const Batch* B) {
-  return B->kernel > 0 ? (int)B->tbl.size() : B->total_steps;
+  if (B->kernel == 0) return B->total_steps;
+  size_t n = 0;
+  for (const auto& t : B->tbl) n = std::max(n, t.size());
+  return (int)n;
 }
 static void issue(Batch* B, int g0, int g1, hipStream_t st) {
-  for (int g = g0; g < g1; ++g) {
-    if (B->kernel > 0) {
-      launch_tb_rec(B, B->tbl[g], st);
-    } else {
-      const Epoch* e = epoch_at(B, g);
-      if (e && e->tiles > 0) launch_step(B, *e, g, st);
+  if (B->kernel > 0) {
+    const int G = B->groups;
+    if (G > 1) {
+      (void)hipEventRecord(B->ev_fork, st);
+      for (int q = 1; q < G; ++q) (void)hipStreamWaitEvent(B->gstream[q], B->ev_fork, 0);
     }
+    for (int g = g0; g < g1; ++g)
+      for (int q = 0; q < G; ++q)
+        if (g < (int)B->tbl[q].size()) launch_tb_rec(B, B->tbl[q][g], q ? B->gstream[q] : st);
+    for (int q = 1; q < G; ++q) {
+      (void)hipEventRecord(B->ev_join[q], B->gstream[q]);
+      (void)hipStreamWaitEvent(st, B->ev_join[q], 0);
+    }
+    return;
+  }
+  for (int g = g0; g < g1; ++g) {
+    const Epoch* e = epoch_at(B, g);
+    if (e && e->tiles > 0) launch_step(B, *e, g, st);
   }
 }
 
@@ -581,6 +642,18 @@ static int batch_create_acoustic(Batch* B, int nshot, const tsw_shot_desc* shots
   B->kernel = kernel;
   if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess)
     return fail((set_error("hipStreamCreate failed"), kErrHip));
+  if (kernel > 0) {
+    B->groups = std::min(launch_groups(), B->slots);
+    B->gstream.assign(B->groups, nullptr);
+    B->ev_join.assign(B->groups, nullptr);
+    B->gstream[0] = B->stream;
+    for (int q = 1; q < B->groups; ++q)
+      if (hipStreamCreateWithFlags(&B->gstream[q], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&B->ev_join[q], hipEventDisableTiming) != hipSuccess)
+        return fail((set_error("launch group stream or event creation failed"), kErrHip));
+    if (B->groups > 1 && hipEventCreateWithFlags(&B->ev_fork, hipEventDisableTiming) != hipSuccess)
+      return fail((set_error("launch group event creation failed"), kErrHip));
+  }
   const float dxf = (float)dx, dzf = (float)dz, dtf = (float)dt;
   B->bc.fs = free_surface_top ? 1 : 0;
   B->bc.n_sponge = n_sponge;
@@ -690,7 +763,7 @@ static int run_acoustic(Batch* B, int mode) {
   bool per_step = true;
   if (mode == TSW_RUN_BATCHED) {
     if (!B->graphs.empty()) {
-      for (auto ex : B->graphs) TSW_HIP(hipGraphLaunch(ex, st));
+      TSW_TRY(replay_graphs(B, st));
     } else {
       issue(B, 0, launches_total(B), st);
     }
@@ -758,9 +831,34 @@ int tsw_batch_time_kernels(tsw_batch_t h, int lane, double* kernel_ms, int* laun
   }
   TSW_ARG(B->nshot > 0, "batch has no acoustic shots");
   hipStream_t st = B->stream;
+  if (B->kernel > 0 && !B->graphs.empty()) {
+    // the blocked kernel as a batched run issues it: the batch's hipGraphs
+    // (the launch groups as parallel branches) inside one event pair
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess ? kOk : kErrHip;
+    float ms = 0.0f;
+    if (rc == kOk && hipEventRecord(e0, st) != hipSuccess) rc = kErrHip;
+    if (rc == kOk) rc = replay_graphs(B, st);
+    if (rc == kOk && hipEventRecord(e1, st) != hipSuccess) rc = kErrHip;
+    if (rc == kOk && (hipEventSynchronize(e1) != hipSuccess ||
+                      hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = kErrHip;
+    (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc != kOk) {
+      set_error("kernel timing run failed");
+      return rc;
+    }
+    *kernel_ms = ms;
+    *launches = launches_total(B);
+    return kOk;
+  }
   // one event pair around each run of kTimedRun back-to-back launches (an
   // event between every two launches would add its own cost to short ones);
-  // the pairs are drained in chunks to bound the pool
+  // the pairs are drained in chunks to bound the pool.  The blocked kernel's
+  // launch groups run a whole run between one fork and one join, as in the
+  // hipGraphs, so consecutive launches overlap as they do there.
   const int chunk = 512;
   std::vector<hipEvent_t> ev(2 * chunk);
   for (auto& e : ev) TSW_HIP(hipEventCreate(&e));
@@ -784,7 +882,11 @@ int tsw_batch_time_kernels(tsw_batch_t h, int lane, double* kernel_ms, int* laun
       if (!e || e->tiles == 0) continue;
     }
     if (inrun == 0 && hipEventRecord(ev[2 * used], st) != hipSuccess) rc = kErrHip;
-    issue(B, g, g + 1, st);
+    if (B->kernel == 0) {
+      issue(B, g, g + 1, st);
+    } else if (inrun == 0) {
+      issue(B, g, std::min(nlaunch, g + kTimedRun), st);
+    }
     ++count;
     if (++inrun == kTimedRun) {
       if (hipEventRecord(ev[2 * used + 1], st) != hipSuccess) rc = kErrHip;

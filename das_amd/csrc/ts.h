@@ -163,7 +163,6 @@ __device__ __forceinline__ void ts_block(const BatchConst& bc, const TbShot& P, 
   const auto cell = [=](int row) __attribute__((always_inline)) {
     return __builtin_amdgcn_readfirstlane((unsigned)(row * nx) * 4u) + cb;
   };
-  const auto clamp_row = [=](int row) __attribute__((always_inline)) { return row < 0 ? 0 : (row < nz ? row : nz - 1); };
   const bool core_col = col_in && col >= cx0 && col < cx0 + W;
   const bool col_int = col >= 2 && col < nx - 2;
   // the strip holds edge columns (cells kept on every row)

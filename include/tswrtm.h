@@ -208,9 +208,14 @@ int tsw_kernel_config(int id, int* T, int* W, int* H, int* threads);
 int tsw_kernel_name(int id, char* buf, int len);
 /* Migrates every shot (mode TSW_RUN_BATCHED or TSW_RUN_PLAIN); synchronous. */
 int tsw_batch_run(tsw_batch_t batch, int mode);
-/* Runs one lane of the batch (0 acoustic, 1 elastic) once with a HIP event
- * pair around each run of 32 back-to-back launches (no graphs); returns the
- * summed time of those runs and the number of launches. */
+/* Runs one lane of the batch (0 acoustic, 1 elastic) once and returns the
+ * kernel time and the number of launches.  The temporally blocked kernels
+ * replay the batch's hipGraphs inside one HIP event pair (without graphs:
+ * one event pair around each run of 32 launches); a launch there is one block
+ * of T steps of every active shot, issued as one kernel per launch group (up
+ * to 8 groups of slots, each on its own stream, forked from and joined back
+ * into the batch's stream).  The per-step kernel is timed without graphs, one
+ * event pair around each run of 32 back-to-back launches. */
 int tsw_batch_time_kernels(tsw_batch_t batch, int lane, double* kernel_ms, int* launches);
 /* Per-shot image and illumination (nz, nx) of the last run; either may be NULL. */
 int tsw_batch_shot_result(tsw_batch_t batch, int shot, float* image_out, float* illum_out);

@@ -122,6 +122,30 @@ def test_blocked_kernels_equal_single_shots(mixed):
                 b.close()
 
 
+@pytest.mark.parametrize("groups,graph_steps", [(1, 100), (3, 5), (8, 0), (8, 20000)])
+def test_launch_groups_equal_single_shots(mixed, monkeypatch, groups, graph_steps):
+    """The blocked kernel's launches split into launch groups on their own
+    streams (TSW_LAUNCH_GROUPS, at most one group per slot): every group
+    count, with no graphs, one launch per graph or the whole run in one graph,
+    gives the single-shot results, and a timed run counts one launch per
+    block of T steps whatever the grouping."""
+    from das_amd.batch import ShotBatch, kernel_configs
+
+    shots, _, _, singles = mixed
+    kid = 25
+    T = kernel_configs()[kid][0]
+    monkeypatch.setenv("TSW_LAUNCH_GROUPS", str(groups))
+    b = ShotBatch(shots, DX, DX, DT, n_sponge=NSP, k=K, slots=0, kernel=kid,
+                  graph_steps=graph_steps)
+    for _ in range(2):
+        b.run("batched")
+        _check(b, shots, singles[True], f"groups={groups} graph_steps={graph_steps}")
+    ms, nl = b.time_kernels()
+    assert ms > 0 and nl == max(2 * -(-s.nt // T) for s in shots)
+    _check(b, shots, singles[True], f"groups={groups} timed run")
+    b.close()
+
+
 def test_mixed_acoustic_elastic_batch(mixed):
     """Elastic shots form their own lane; acoustic and elastic shots of one
     batch each equal their single-shot migration bit for bit."""
