@@ -680,7 +680,8 @@ def main():
         from das_amd import autotune
 
         t0 = time.time()
-        tuning = autotune.tune(cfg.shots, cfg.dx, cfg.dt, cfg.k, cfg.n_sponge, log=log)
+        tuning = autotune.tune(cfg.shots, cfg.dx, cfg.dt, cfg.k, cfg.n_sponge, log=log,
+                               graph_steps=args.graph_steps)
         log(f"autotune ({time.time() - t0:.1f}s): {tuning}")
         if args.kernel < 0:
             args.kernel = tuning["kernel"]

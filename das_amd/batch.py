@@ -79,7 +79,7 @@ class ShotBatch:
 
     def __init__(self, shots: Sequence, dx: float, dz: float, dt: float, n_sponge: int = 30,
                  sponge_alpha: float = 0.0015, free_surface_top: bool = True, k: int = 2,
-                 slots: int = 0, graph_steps: int = 100, kernel: int = 0):
+                 slots: int = 0, graph_steps: int = 20000, kernel: int = 0):
         self.shots = list(shots)
         self.dx, self.dz, self.dt, self.k = float(dx), float(dz), float(dt), int(k)
         keep = []

@@ -188,7 +188,8 @@ int tsw_migrate_shot_elastic(tsw_grid_t grid, const float* const* model_fwd,
  * (kind TSW_SHOT_ELASTIC, vel_* unused) the elastic lane; the two lanes run on
  * their own streams and overlap.  Uploads every shot's models, source and data, sizes the HBM state
  * and captures the time loop into hipGraphs of `graph_steps` time steps
- * (0: no graphs).  Shots are packed into `slots` concurrent lanes (<= 0: all
+ * (0: no graphs; every graph of a blocked kernel ends with a join of its
+ * launch groups, so fewer, larger graphs run faster).  Shots are packed into `slots` concurrent lanes (<= 0: all
  * shots at once).  kernel 0: one launch per time step over the tiles of every
  * active shot; kernel c in [1, tsw_kernel_configs()]: temporally blocked
  * launches of configuration c (T steps per launch).  The elastic lane runs
