@@ -32,6 +32,7 @@
 // step computes anyway) rather than u^n: the gradient term then reads one
 // value per cell instead of a 9-point stencil.
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 #include <cstring>
 #include <vector>
@@ -957,8 +958,27 @@ struct Fwi {
   int atb = 0;         // FP32 blocked adjoint: configuration kFwiAdjTb[atb - 1]; 0 = per step
   DevBuf sets, table;  // blocked forward: two state sets [2][3][cmax][n], sponge table
   DevBuf asets;        // blocked adjoint: two state sets [2][3][cmax][n]
+  // launch groups of the blocked passes: a group's shots split into `views`
+  // contiguous sets, each set's launches on its own stream (set 0 on the
+  // call's stream), forked from and joined back into the call's stream, so
+  // that one set's launch start and tail overlap the other's work.  The
+  // 16-shot 300x800 gradient took 0.1605 s in one set, 0.1455 s in 2, 0.186 s
+  // in 4 and 0.196 s in 8 (the passes launch without graphs, and 4 or 8 sets
+  // of short launches outrun the host's launch rate); results bit-identical.
+  // The environment variable TSW_FWI_GROUPS (1..kFwiViews) sets the count.
+  static constexpr int kFwiViews = 8;
+  int views = 2;
+  hipStream_t vs[kFwiViews] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[kFwiViews] = {};
   double last_loss = 0;
   bool have_forward = false;
+  ~Fwi() {
+    for (int v = 1; v < kFwiViews; ++v) {
+      if (vs[v]) (void)hipStreamDestroy(vs[v]);
+      if (ev_join[v]) (void)hipEventDestroy(ev_join[v]);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+  }
   size_t esz() const { return prec == 8 ? 8 : 4; }
   long long cells() const { return (long long)nx * nz; }
 };
@@ -1053,23 +1073,61 @@ static FwiTbArgs tb_base(const Fwi* F, const FwiGroup& G) {
 // the receivers or saving L u^it of every step into `save` (segment starting at
 // save_it0); blocks never cross a multiple of `cut` (checkpoint boundaries,
 // 0: none) so that the state is in memory there.
-static void tb_run(const Fwi* F, const FwiGroup& G, int i0, int i1, int* k, bool rec, float* save,
-                   int save_it0, hipStream_t st) {
+// A contiguous set of a group's shots: positions [p0, p0 + cnt).  Every
+// per-shot array of the blocked passes is indexed by position (shot j at
+// + j * n), so a view is the same launch with its pointers moved by p0.
+struct FwiView {
+  int p0, cnt;
+};
+static std::vector<FwiView> views_of(const Fwi* F, const FwiGroup& G) {
+  const int nv = std::max(1, std::min(F->views, G.count()));
+  std::vector<FwiView> v(nv);
+  for (int i = 0; i < nv; ++i) {
+    v[i].p0 = (int)((long long)i * G.count() / nv);
+    v[i].cnt = (int)((long long)(i + 1) * G.count() / nv) - v[i].p0;
+  }
+  return v;
+}
+static hipStream_t view_stream(const Fwi* F, int v, hipStream_t st) { return v ? F->vs[v] : st; }
+static int views_fork(Fwi* F, int nv, hipStream_t st) {
+  if (nv <= 1) return kOk;
+  TSW_HIP(hipEventRecord(F->ev_fork, st));
+  for (int v = 1; v < nv; ++v) TSW_HIP(hipStreamWaitEvent(F->vs[v], F->ev_fork, 0));
+  return kOk;
+}
+static int views_join(Fwi* F, int nv, hipStream_t st) {
+  for (int v = 1; v < nv; ++v) {
+    TSW_HIP(hipEventRecord(F->ev_join[v], F->vs[v]));
+    TSW_HIP(hipStreamWaitEvent(st, F->ev_join[v], 0));
+  }
+  return kOk;
+}
+
+// Blocked steps [i0, i1) of the views V of a group, each on its stream, the
+// views' launches interleaved; state set *k (updated) as for one launch list.
+static void tb_run(const Fwi* F, const FwiGroup& G, const std::vector<FwiView>& V, int i0, int i1,
+                   int* k, bool rec, float* save, int save_it0, hipStream_t st) {
   FwiTbArgs a = tb_base(F, G);
   const FwiTbCfg& c = kFwiTb[F->tb - 1];
+  const long long n = F->cells();
   a.rec = rec ? 1 : 0;
-  a.save = save;
   a.save_it0 = save_it0;
-  a.sstride = (long long)G.count() * F->cells();
+  a.sstride = (long long)G.count() * n;
   for (int it = i0; it < i1;) {
     const int len = std::min(c.T, i1 - it);
-    for (int l = 0; l < 3; ++l) {
-      a.rd[l] = tb_set(F, *k, l);
-      a.wr[l] = tb_set(F, *k ^ 1, l);
-    }
     a.it0 = it;
     a.nsub = len;
-    c.launch(a, G.count(), st);
+    for (size_t v = 0; v < V.size(); ++v) {
+      const long long off = (long long)V[v].p0 * n;
+      for (int l = 0; l < 3; ++l) {
+        a.rd[l] = tb_set(F, *k, l) + off;
+        a.wr[l] = tb_set(F, *k ^ 1, l) + off;
+      }
+      a.save = save ? save + off : nullptr;
+      a.meta = (const FwiMeta*)G.meta.p + V[v].p0;
+      a.count = V[v].cnt;
+      c.launch(a, V[v].cnt, view_stream(F, (int)v, st));
+    }
     *k ^= 1;
     it += len;
   }
@@ -1098,18 +1156,25 @@ static int fwi_forward_t(Fwi* F, const T* d_vel, hipStream_t st, bool store_all)
     if (std::is_same<T, float>::value && F->tb > 0) {
       int k = 0;
       tb_reset(F, G, k, st);
+      const std::vector<FwiView> V = views_of(F, G);
+      TSW_TRY(views_fork(F, (int)V.size(), st));
       for (int it = 0; it < G.nt;) {
         int end = G.nt;
         if (!store_all) {
           T* ck = (T*)G.ckpt.p + (long long)(it / K) * 3 * cn;  // it % K == 0 here
-          for (int l = 0; l < 3; ++l)
-            TSW_HIP(hipMemcpyAsync(ck + l * cn, tb_set(F, k, l), cn * sizeof(T),
-                                   hipMemcpyDeviceToDevice, st));
+          for (size_t v = 0; v < V.size(); ++v) {
+            const long long off = (long long)V[v].p0 * n;
+            for (int l = 0; l < 3; ++l)
+              TSW_HIP(hipMemcpyAsync(ck + l * cn + off, tb_set(F, k, l) + off,
+                                     V[v].cnt * n * sizeof(T), hipMemcpyDeviceToDevice,
+                                     view_stream(F, (int)v, st)));
+          }
           end = std::min(G.nt, it + K);
         }
-        tb_run(F, G, it, end, &k, true, nullptr, 0, st);
+        tb_run(F, G, V, it, end, &k, true, nullptr, 0, st);
         it = end;
       }
+      TSW_TRY(views_join(F, (int)V.size(), st));
     } else {
     fwd_reset<T>(F, G, st);
     FwdArgs<T> s = fwd_base<T>(F, G);
@@ -1154,8 +1219,8 @@ static float* atb_set(const Fwi* F, int k, int l) {
 
 // Blocked adjoint steps n1-1 down to n0 of a group from state set *k
 // (updated); `lu` holds L u^it of the segment starting at n0.
-static void adj_tb_run(const Fwi* F, const FwiGroup& G, int n0, int n1, int* k, const float* lu,
-                       hipStream_t st) {
+static void adj_tb_run(const Fwi* F, const FwiGroup& G, const std::vector<FwiView>& V, int n0,
+                       int n1, int* k, const float* lu, hipStream_t st) {
   const FwiAdjTbCfg& c = kFwiAdjTb[F->atb - 1];
   FwiAdjTbArgs a;
   std::memset(&a, 0, sizeof a);
@@ -1178,15 +1243,23 @@ static void adj_tb_run(const Fwi* F, const FwiGroup& G, int n0, int n1, int* k, 
   const int W = 64 * c.WX - 4 * c.T, H = c.WZ * c.R - 4 * c.T;
   a.tiles_x = (F->nx + W - 1) / W;
   a.tiles = a.tiles_x * ((F->nz + H - 1) / H);
+  const long long n = F->cells();
   for (int it = n1 - 1; it >= n0;) {
     const int len = std::min(c.T, it - n0 + 1);
-    for (int l = 0; l < 3; ++l) {
-      a.rd[l] = atb_set(F, *k, l);
-      a.wr[l] = atb_set(F, *k ^ 1, l);
-    }
     a.it0 = it;
     a.nsub = len;
-    c.launch(a, st);
+    for (size_t v = 0; v < V.size(); ++v) {
+      const long long off = (long long)V[v].p0 * n;
+      for (int l = 0; l < 3; ++l) {
+        a.rd[l] = atb_set(F, *k, l) + off;
+        a.wr[l] = atb_set(F, *k ^ 1, l) + off;
+      }
+      a.lu = lu + off;
+      a.grad = (float*)F->gC.p + off;
+      a.meta = (const FwiMeta*)G.meta.p + V[v].p0;
+      a.count = V[v].cnt;
+      c.launch(a, view_stream(F, (int)v, st));
+    }
     *k ^= 1;
     it -= len;
   }
@@ -1239,6 +1312,11 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
     FwdArgs<T> s = fwd_base<T>(F, G);
     const int K = store_all ? nt : F->kint;
     const int nseg = (nt + K - 1) / K;
+    // the blocked recompute and the blocked adjoint run the group's shots as
+    // views on their own streams from here to the join below
+    const bool views = std::is_same<T, float>::value && F->tb > 0 && blocked_adj && !store_all;
+    const std::vector<FwiView> V = views ? views_of(F, G) : std::vector<FwiView>{{0, cnt}};
+    if (views) TSW_TRY(views_fork(F, (int)V.size(), st));
     for (int sg = nseg - 1; sg >= 0; --sg) {
       const int n0 = sg * K, n1 = std::min(nt, n0 + K);
       if (std::is_same<T, float>::value && F->tb > 0) {
@@ -1247,11 +1325,15 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
           tb_reset(F, G, k, st);
         } else {
           const T* ck = (const T*)G.ckpt.p + (long long)sg * 3 * cn;
-          for (int l = 0; l < 3; ++l)
-            TSW_HIP(hipMemcpyAsync(tb_set(F, k, l), ck + l * cn, cn * sizeof(T),
-                                   hipMemcpyDeviceToDevice, st));
+          for (size_t v = 0; v < V.size(); ++v) {
+            const long long off = (long long)V[v].p0 * n;
+            for (int l = 0; l < 3; ++l)
+              TSW_HIP(hipMemcpyAsync(tb_set(F, k, l) + off, ck + l * cn + off,
+                                     V[v].cnt * n * sizeof(T), hipMemcpyDeviceToDevice,
+                                     view_stream(F, (int)v, st)));
+          }
         }
-        tb_run(F, G, n0, n1, &k, false, (float*)F->seg.p, n0, st);
+        tb_run(F, G, V, n0, n1, &k, false, (float*)F->seg.p, n0, st);
       } else {
       if (store_all) {
         fwd_reset<T>(F, G, st);  // one segment: the whole forward again
@@ -1269,7 +1351,7 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
         fwd_step<T>(F, G, s, it, (T*)F->seg.p + (long long)(it - n0) * cn, false, st);
       }
       if (blocked_adj) {
-        adj_tb_run(F, G, n0, n1, &ak, (const float*)F->seg.p, st);
+        adj_tb_run(F, G, V, n0, n1, &ak, (const float*)F->seg.p, st);
         continue;
       }
       for (int it = n1 - 1; it >= n0;) {
@@ -1294,6 +1376,7 @@ static int fwi_backward_t(Fwi* F, const T* d_vel, T* d_grad, hipStream_t st, boo
         }
       }
     }
+    if (views) TSW_TRY(views_join(F, (int)V.size(), st));
     TSW_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_fwi_chain<T>, dim3(g1(n)), dim3(256), 0, st, (const T*)F->gC.p, F->cmax,
@@ -1343,6 +1426,16 @@ int tsw_fwi_create(int nx, int nz, double dx, double dz, double dt, int n_sponge
   F->fs = free_surface_top ? 1 : 0;
   F->prec = precision;
   F->n_sponge = n_sponge;
+  if (const char* ev = std::getenv("TSW_FWI_GROUPS")) {
+    const int v = std::atoi(ev);
+    if (v >= 1 && v <= Fwi::kFwiViews) F->views = v;
+  }
+  if (hipEventCreateWithFlags(&F->ev_fork, hipEventDisableTiming) != hipSuccess)
+    return fail((set_error("event creation failed"), kErrHip));
+  for (int v = 1; v < Fwi::kFwiViews; ++v)
+    if (hipStreamCreateWithFlags(&F->vs[v], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&F->ev_join[v], hipEventDisableTiming) != hipSuccess)
+      return fail((set_error("stream creation failed"), kErrHip));
   F->alpha = sponge_alpha;
   const long long n = F->cells();
   const size_t es = F->esz();

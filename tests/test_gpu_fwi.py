@@ -380,3 +380,42 @@ def test_blocked_adjoint_equals_per_step(config, fs):
             np.testing.assert_array_equal(a, b)
     with pytest.raises(ValueError):
         _op(shots[:1], np.float64).set_adjoint_blocking(1)
+
+
+@pytest.mark.parametrize("groups", [2, 3, 8])
+def test_launch_groups_equal_one_group(monkeypatch, groups):
+    """The blocked passes split a group's shots into launch groups on their
+    own streams (TSW_FWI_GROUPS): loss, predicted data, gradient and adjoint
+    at the sources equal one group's bit for bit, with checkpoints and with
+    every level kept, for five shots of one length (sets of uneven size) and
+    one of another."""
+    from das_amd.fwi import MisfitShot
+    from das_amd.wave_solver import ricker_wavelet
+
+    v_true, v0 = _models()
+    src, _ = ricker_wavelet(30.0, DT, NT * DT)
+    src = src.astype(np.float32)
+    geo = [(20, 2, np.arange(0, NX, 2), 2), (1, 1, np.arange(NX), 0), (NX - 2, NZ - 3, np.arange(3, NX - 1, 5), 1),
+           (40, 5, np.array([7, 7, 30]), 3), (33, 0, np.arange(5, NX, 7), 2), (12, 4, np.arange(NX - 9, NX), 1)]
+    shots = [MisfitShot(src=src if i != 5 else src[:NT - 17], six=a, siz=b, rix=r.astype(np.int32), riz=z,
+                        d_obs=np.zeros((NT if i != 5 else NT - 17, len(r)), np.float32))
+             for i, (a, b, r, z) in enumerate(geo)]
+    op = _op(shots, np.float32)
+    op.forward_loss(_dev(v_true, np.float32))
+    for i, sh in enumerate(shots):
+        sh.d_obs = op.predicted(i)
+    op.close()
+    v = _dev(v0, np.float32)
+    for ck in (0, 7):
+        out = []
+        for g in (1, groups):
+            monkeypatch.setenv("TSW_FWI_GROUPS", str(g))
+            op = _op(shots, np.float32, ckpt=ck)
+            loss = op.forward_loss(v)
+            grad = op.gradient(v).cpu().numpy()
+            out.append([np.float64(loss), grad] + [op.predicted(i) for i in range(len(shots))]
+                       + [op.adjoint_at_source(i) for i in range(len(shots))])
+            op.close()
+        assert np.abs(out[0][1]).max() > 0
+        for a, b in zip(*out):
+            np.testing.assert_array_equal(a, b)
