@@ -361,7 +361,7 @@ static const Epoch* epoch_at(const Batch* B, int g) {
   return nullptr;
 }
 
-static void issue(Batch* B, int g0, int g1, hipStream_t st);
+static int issue(Batch* B, int g0, int g1, hipStream_t st);
 static int launches_total(const Batch* B);
 
 // hipGraphs of `graph_steps` time steps each (graph_steps/T blocked launches).
@@ -377,9 +377,15 @@ static int capture_graphs(Batch* B) {
   for (int g0 = 0; g0 < total; g0 += per) {
     const int g1 = std::min(total, g0 + per);
     TSW_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-    issue(B, g0, g1, B->stream);
-    hipGraph_t gr;
-    TSW_HIP(hipStreamEndCapture(B->stream, &gr));
+    const int rc = issue(B, g0, g1, B->stream);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(B->stream, &gr);
+    if (rc != kOk || e != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      if (rc != kOk) return rc;
+      set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+      return kErrHip;
+    }
     B->graph_defs.push_back(gr);
     hipGraphExec_t ex;
     TSW_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
@@ -527,26 +533,27 @@ static int launches_total(const Batch* B) {
   for (const auto& t : B->tbl) n = std::max(n, t.size());
   return (int)n;
 }
-static void issue(Batch* B, int g0, int g1, hipStream_t st) {
+static int issue(Batch* B, int g0, int g1, hipStream_t st) {
   if (B->kernel > 0) {
     const int G = B->groups;
     if (G > 1) {
-      (void)hipEventRecord(B->ev_fork, st);
-      for (int q = 1; q < G; ++q) (void)hipStreamWaitEvent(B->gstream[q], B->ev_fork, 0);
+      TSW_HIP(hipEventRecord(B->ev_fork, st));
+      for (int q = 1; q < G; ++q) TSW_HIP(hipStreamWaitEvent(B->gstream[q], B->ev_fork, 0));
     }
     for (int g = g0; g < g1; ++g)
       for (int q = 0; q < G; ++q)
         if (g < (int)B->tbl[q].size()) launch_tb_rec(B, B->tbl[q][g], q ? B->gstream[q] : st);
     for (int q = 1; q < G; ++q) {
-      (void)hipEventRecord(B->ev_join[q], B->gstream[q]);
-      (void)hipStreamWaitEvent(st, B->ev_join[q], 0);
+      TSW_HIP(hipEventRecord(B->ev_join[q], B->gstream[q]));
+      TSW_HIP(hipStreamWaitEvent(st, B->ev_join[q], 0));
     }
-    return;
+    return kOk;
   }
   for (int g = g0; g < g1; ++g) {
     const Epoch* e = epoch_at(B, g);
     if (e && e->tiles > 0) launch_step(B, *e, g, st);
   }
+  return kOk;
 }
 
 }  // namespace tsw
@@ -765,7 +772,7 @@ static int run_acoustic(Batch* B, int mode) {
     if (!B->graphs.empty()) {
       TSW_TRY(replay_graphs(B, st));
     } else {
-      issue(B, 0, launches_total(B), st);
+      TSW_TRY(issue(B, 0, launches_total(B), st));
     }
     per_step = B->kernel == 0;
   } else {
@@ -882,10 +889,10 @@ int tsw_batch_time_kernels(tsw_batch_t h, int lane, double* kernel_ms, int* laun
       if (!e || e->tiles == 0) continue;
     }
     if (inrun == 0 && hipEventRecord(ev[2 * used], st) != hipSuccess) rc = kErrHip;
-    if (B->kernel == 0) {
-      issue(B, g, g + 1, st);
-    } else if (inrun == 0) {
-      issue(B, g, std::min(nlaunch, g + kTimedRun), st);
+    if (rc == kOk && B->kernel == 0) {
+      rc = issue(B, g, g + 1, st);
+    } else if (rc == kOk && inrun == 0) {
+      rc = issue(B, g, std::min(nlaunch, g + kTimedRun), st);
     }
     ++count;
     if (++inrun == kTimedRun) {
